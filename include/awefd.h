@@ -1,0 +1,144 @@
+/*
+ * awefd.h -- C ABI of libawefd, the MI355X (gfx950) acoustic finite-difference
+ * time loop behind the Madagascar programs sfAWEFDcart2D, sfAWEFDcart3D,
+ * sfAWEFDspher2D and sfAWEFDspher3D (Nacnud04/wavefield_imaging, CODE/).
+ *
+ * The reference has no library boundary: each program's main() reads RSF
+ * files, sets up the grid and runs the time loop in-process.  libawefd is that
+ * time loop behind plain C entry points (no torch or HIP types in the
+ * signatures), so that the project's own C programs (the drop-in CLIs) and a
+ * ctypes binding share it.  Each entry point names the part of the reference
+ * main it replaces.
+ *
+ * Conventions
+ *   - Return value 0 on success, a negative AWEFD_E* code on error; the message
+ *     is available from awefd_last_error() (thread-local).  Nothing exits.
+ *   - Host buffers belong to the caller.  Device buffers belong to the handle,
+ *     except in the *_dev entry points, which take device pointers.
+ *   - One handle = one grid on one GPU, one HIP stream.  A handle is not
+ *     thread-safe; separate handles are independent.
+ *   - Arrays are in RSF order, axis 1 fastest: cartesian (z, x, y),
+ *     polar/spherical (r, theta, phi).
+ */
+#ifndef AWEFD_H
+#define AWEFD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AWEFD_CART2D  0   /* sfAWEFDcart2D   MAWEFDcart2Dcpu.c  */
+#define AWEFD_CART3D  1   /* sfAWEFDcart3D   MAWEFDcart3Dcpu.c  */
+#define AWEFD_SPHER2D 2   /* sfAWEFDspher2D  MAWEFDspher2Dcpu.c */
+#define AWEFD_SPHER3D 3   /* sfAWEFDspher3D  MAWEFDspher3Dcpu.c */
+
+#define AWEFD_OK        0
+#define AWEFD_EINVAL   -1   /* bad argument or unsupported option            */
+#define AWEFD_EHIP     -2   /* HIP runtime error                             */
+#define AWEFD_ENOMEM   -3   /* host or device allocation failed              */
+#define AWEFD_ESTATE   -4   /* call out of order (e.g. run before model set) */
+
+/* Program parameters (the getpar keys of the four mains, same meaning and
+ * defaults; MAWEFDcart2Dcpu.c:116-128,167-211, MAWEFDcart3Dcpu.c:116-210,
+ * MAWEFDspher2Dcpu.c:56-116, MAWEFDspher3Dcpu.c:57-133). */
+typedef struct awefd_params {
+    int   kind;      /* AWEFD_CART2D ... AWEFD_SPHER3D                        */
+    int   n[3];      /* model axes n1,n2,n3 (unpadded); n3 = 1 in 2-D          */
+    float o[3];      /* axis origins o1,o2,o3                                 */
+    float d[3];      /* axis samplings d1,d2,d3                               */
+    int   nt;        /* time steps (wavelet axis 2)                           */
+    float dt;        /* time sampling                                         */
+    int   nb;        /* nb=  (< 0: not given; clamped to the stencil half-width) */
+    int   fdorder;   /* fdorder= (cartesian; 2..12, default 4)                */
+    int   optfd;     /* optfd=  (cartesian)                                   */
+    int   dabc;      /* dabc=   (cartesian sponge, plus the 3-D one-way ABC)   */
+    int   fsrf;      /* free= / fsrf=                                         */
+    int   expl;      /* expl=  cartesian: one wavelet sample for all sources;
+                        spherical: reflectivity-scaled injection (default y)   */
+    int   cden;      /* cden=  (cartesian constant density)                    */
+    int   sse;       /* operation order of ts_kernel.c: 1 = 4-wide SSE build
+                        (default), 0 = scalar build                           */
+    int   jdata;     /* jdata= (receiver decimation; traces keep every step)   */
+    int   device;    /* gpu=   HIP device ordinal                             */
+} awefd_params;
+
+typedef struct awefd_model awefd_model;
+
+/* Fill *p with the reference defaults of program `kind` (MAWEFD*cpu.c getpar
+ * calls).  n/o/d/nt/dt must still be set by the caller. */
+int awefd_params_default(int kind, awefd_params *p);
+
+/* Grid set-up: replaces fdutil_init/fdutil3d_init, compute_fdcoef and the
+ * allocations of MAWEFDcart2Dcpu.c:209-287 (3-D :207-284, polar :114-168,
+ * spherical :99-206).  Allocates the device wavefields on p->device. */
+int awefd_create(const awefd_params *p, awefd_model **out);
+
+/* Padded grid dimensions (fastest first) and the padding width nb. */
+int awefd_padded_dims(const awefd_model *m, int dims[3], int *nb);
+
+/* Velocity (and density, nullable) in RSF order, unpadded: replaces the
+ * sf_floatread + expand/expand3d/expand_cpu_* and the (v dt)^2 scaling,
+ * free-surface zeroing, sponge and one-way coefficient set-up
+ * (MAWEFDcart2Dcpu.c:258-295, MAWEFDcart3Dcpu.c:254-293,
+ * MAWEFDspher2Dcpu.c:153-220, MAWEFDspher3Dcpu.c:185-272). */
+int awefd_set_model(awefd_model *m, const float *vel, const float *den);
+
+/* Source / receiver coordinates as stored in the sou= / rec= files:
+ * cart2d (x,z), cart3d (x,y,z), spher2d (r,theta), spher3d (r,phi,theta).
+ * Replaces pt2dread1/pt3dread1 + lint2d_make/lint3d_make
+ * (MAWEFDcart2Dcpu.c:249-256, MAWEFDcart3Dcpu.c:245-252, fdutil.c:1363-1539). */
+int awefd_set_sources(awefd_model *m, int ns, const float *coords);
+int awefd_set_receivers(awefd_model *m, int nr, const float *coords);
+
+/* Wavelet stream in the order the time loop reads it: cartesian reads
+ * (expl ? 1 : ns) samples per step (MAWEFDcart2Dcpu.c:318-326); spherical
+ * reads one trace of nt samples (MAWEFDspher3Dcpu.c:173-182).
+ * nsamples = number of floats available. */
+int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav);
+
+/* Snapshot callback: the padded field the reference would write for step it
+ * (u0 after the swap / h_po after the step), host memory, valid during the
+ * call only. */
+typedef void (*awefd_snap_fn)(void *user, int it, const float *field, const int dims[3]);
+
+/* Run the whole time loop (MAWEFDcart2Dcpu.c:297-383, MAWEFDcart3Dcpu.c:295-366,
+ * MAWEFDspher2Dcpu.c:264-317, MAWEFDspher3Dcpu.c:302-359).
+ * traces: host nt*nr floats (row it = receivers at step it), may be NULL.
+ * cb is called for it % jsnap == 0 when jsnap > 0 (cb may be NULL). */
+int awefd_run(awefd_model *m, float *traces, int jsnap, awefd_snap_fn cb, void *user);
+
+/* Same loop with every buffer already resident in HBM: d_traces is a device
+ * pointer to nt*nr floats (may be NULL); no host traffic, no snapshots. */
+int awefd_run_dev(awefd_model *m, float *d_traces);
+
+/* Reset the wavefields to zero (the state before the first step). */
+int awefd_reset(awefd_model *m);
+
+/* Copy the current padded field, as the reference would write it after the
+ * last completed step, into host memory (dims from awefd_padded_dims). */
+int awefd_get_field(awefd_model *m, float *host);
+
+/* Bit-exact view of the host index map (lint2d_make / lint3d_make,
+ * fdutil.c:1363-1539): which = 0 sources, 1 receivers.  idx gets 3 ints per
+ * point (j1, j2, j3 along the fastest..slowest axis), w gets 8 floats per
+ * point in the reference corner order (4 used in 2-D). */
+int awefd_debug_points(const awefd_model *m, int which, int *idx, float *w);
+
+/* Per-kernel timing with HIP events on the handle's stream.  enable=1 starts
+ * recording events around every launch of the dominant (stencil) kernel;
+ * awefd_kernel_stats returns the launch count and total milliseconds. */
+int awefd_profile(awefd_model *m, int enable);
+int awefd_kernel_stats(awefd_model *m, long *launches, double *total_ms);
+
+/* Device the handle runs on, and the raw device pointer of its stream. */
+int awefd_device(const awefd_model *m);
+
+const char *awefd_last_error(void);
+void awefd_destroy(awefd_model *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,279 @@
+/*
+ * CPU oracle (test infrastructure only) for the acoustic FD time loop.
+ * Independent C restatement of the reference CPU programs; see awefd_oracle.h.
+ *
+ * Shared set-up helpers live here; the propagation loops are instantiated
+ * twice from oracle_prop.inc (REAL=float -> *_f32, REAL=double -> *_f64).
+ *
+ * Build: gcc -O2 -fPIC -ffp-contract=off -fno-fast-math -shared (oracle/Makefile).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "awefd_oracle.h"
+
+#define ORC_PI 3.14159265358979323846264338328
+
+/* ------------------------------------------------------------------ */
+/* FD coefficients: MAWEFDcart2Dcpu.c:423-551 (3-D: MAWEFDcart3Dcpu.c:404-531) */
+
+static int orc_factorial(int n)          /* MAWEFDcart2Dcpu.c:476-484 */
+{
+    int r = 1;
+    for (int i = 2; i <= n; i++) r *= i;
+    return r;
+}
+
+/* Published optimized coefficient tables (Liu 2013; Zhang & Yao 2013) used by
+ * optfd=y, MAWEFDcart2Dcpu.c:508-549.  Stored as the reference stores them:
+ * single-precision values of the printed decimals. */
+static const float orc_opt_c[11][11] = {
+    {0},
+    {0.f, 1.f},
+    {0.f, 1.369074f, -(float)0.09266816},
+    {0.f, 1.573661f, -0.1820268f, 0.01728053f},
+    {0.f, 1.700010f, -0.2554615f, 0.04445392f, -0.004946851f},
+    {0.f, 1.782836f, -0.3124513f, 0.07379487f, -0.01532122f, 0.001954439f},
+    {0.f, 1.837023f, -0.3538895f, 0.09978343f, -0.02815486f, 0.006556587f, -0.0009405699f},
+    {0.f, 1.874503f, -0.3845794f, 0.1215162f, -0.04121749f, 0.01295522f, -0.003313813f,
+     0.0005310053f},
+    {0.f, 1.901160f, -0.4074304f, 0.1390909f, -0.05318775f, 0.02004823f, -0.006828249f,
+     0.001895771f, -0.0003369052f},
+    {0.f, 1.919909f, -0.4240446f, 0.1526043f, -0.06322328f, 0.02676005f, -0.01080739f,
+     0.003907747f, -0.001158024f, 0.0002240247f},
+    {0.f, 1.918204f, -0.4225858f, 0.1514992f, -0.06249474f, 0.02637196f, -0.01066631f,
+     0.003915625f, -0.001219872f, 0.0002863976f, -0.00003744830f},
+};
+static const float orc_opt_b[7][7] = {
+    {0},
+    {0.0f, 0.5f},
+    {0.0f, (float)0.67880327, -(float)0.08962729},
+    {0.0f, (float)0.77793115, -(float)0.17388691, (float)0.02338713},
+    {0.0f, (float)0.84149635, -(float)0.24532989, (float)0.06081891, -(float)0.00839807},
+    {0.0f, (float)0.88414717, -(float)0.30233648, (float)0.10275057, -(float)0.02681517,
+     (float)0.00398089},
+    {0.0f, (float)0.91067892, -(float)0.34187892, (float)0.13833962, -(float)0.04880710,
+     (float)0.01302148, -(float)0.00199047},
+};
+
+/* Binomial-window coefficients, MAWEFDcart2Dcpu.c:486-500: float quotient,
+ * times a double cosine and integer factorials, rounded back to float. */
+static void orc_normal_coef(int nop, int order, float *cc)
+{
+    int hf = orc_factorial(nop);
+    hf *= hf;
+    cc[0] = 0.f;
+    for (int n = 1; n <= nop; n++) {
+        float head = -2.f / (float)(n * n);
+        double v = (double)head * cos(n * ORC_PI) * (double)hf
+                   / (double)orc_factorial(nop + n) / (double)orc_factorial(nop - n);
+        cc[n] = (float)v;
+        if (order == 1) cc[n] = cc[n] * (float)n / 2.f;
+    }
+}
+
+int orc_fdcoef(int nop, int ndim, const float *d, int optfd, int order, float *out)
+{
+    float c[11];
+    if (nop < 1 || nop > 6 || ndim < 2 || ndim > 3) return -1;
+    if (order == 2) {
+        if (optfd) memcpy(c, orc_opt_c[nop], sizeof(float) * (nop + 1));
+        else       orc_normal_coef(nop, 2, c);
+        out[0] = 0.f;
+        for (int a = 0; a < ndim; a++) {
+            float h2 = d[a] * d[a];
+            for (int k = 1; k <= nop; k++) {
+                out[a * nop + k] = c[k] / h2;
+                out[0] += out[a * nop + k];
+            }
+        }
+        out[0] *= -2.0f;
+    } else {
+        if (optfd) memcpy(c, orc_opt_b[nop], sizeof(float) * (nop + 1));
+        else       orc_normal_coef(nop, 1, c);
+        out[0] = 0.f;
+        for (int a = 0; a < ndim; a++)
+            for (int k = 1; k <= nop; k++) out[a * nop + k] = c[k] / d[a];
+    }
+    return 0;
+}
+
+/* Double-precision coefficients for the FP64 oracle (same formulas in double). */
+static int orc_fdcoef_d(int nop, int ndim, const float *d, int optfd, int order, double *out)
+{
+    double c[11];
+    if (nop < 1 || nop > 6) return -1;
+    int hf = orc_factorial(nop);
+    hf *= hf;
+    for (int n = 1; n <= nop; n++) {
+        if (optfd) c[n] = order == 2 ? orc_opt_c[nop][n] : orc_opt_b[nop][n];
+        else {
+            c[n] = -2.0 / (n * n) * cos(n * ORC_PI) * hf
+                   / orc_factorial(nop + n) / orc_factorial(nop - n);
+            if (order == 1) c[n] = c[n] * n / 2.0;
+        }
+    }
+    out[0] = 0.0;
+    for (int a = 0; a < ndim; a++)
+        for (int k = 1; k <= nop; k++) {
+            out[a * nop + k] = order == 2 ? c[k] / ((double)d[a] * d[a]) : c[k] / d[a];
+            if (order == 2) out[0] += out[a * nop + k];
+        }
+    out[0] *= -2.0;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Gaussian sponge table, fdutil.c:2181-2203:
+ * sb = (float)(4.0*nb); fb = (float)(ib/(sqrt(2)*sb)); w = (float)exp(-fb*fb). */
+void orc_sponge_table(int nb, float *w)
+{
+    float sb = (float)(4.0 * nb);
+    for (int ib = 0; ib < nb; ib++) {
+        float fb = (float)(ib / (sqrt(2.0) * (double)sb));
+        float e = -fb * fb;
+        w[ib] = (float)exp((double)e);
+    }
+}
+
+/* Spherical sponge factor, spher_utils.c:665-667 (double throughout). */
+double orc_spher_damp(int i, int nb)
+{
+    float alpha = 0.90f;
+    return exp(-1.0 * fabs((pow((i - 1.0), 2) * log((double)alpha)) / (pow(nb, 2))));
+}
+
+/* ------------------------------------------------------------------ */
+/* Bilinear / trilinear index maps, fdutil.c:1363-1411 and 1474-1539.
+ * Float arithmetic, truncation toward zero, out-of-grid points pinned to
+ * node 0 with the whole weight on the second z node. */
+void orc_lint2d(int na, const float *xz, int nzpad, int nxpad, float ozpad,
+                float oxpad, float dz, float dx, int *jz, int *jx, float *w)
+{
+    for (int a = 0; a < na; a++) {
+        float x = xz[2 * a], z = xz[2 * a + 1];
+        float f1, f2;
+        float zmax = ozpad + (float)(nzpad - 1) * dz;
+        float xmax = oxpad + (float)(nxpad - 1) * dx;
+        if (z >= ozpad && z < zmax && x >= oxpad && x < xmax) {
+            jz[a] = (int)((z - ozpad) / dz);
+            jx[a] = (int)((x - oxpad) / dx);
+            f1 = (z - ozpad) / dz - (float)jz[a];
+            f2 = (x - oxpad) / dx - (float)jx[a];
+        } else {
+            jz[a] = 0; jx[a] = 0; f1 = 1; f2 = 0;
+        }
+        w[4 * a + 0] = (1 - f1) * (1 - f2);
+        w[4 * a + 1] = (f1) * (1 - f2);
+        w[4 * a + 2] = (1 - f1) * (f2);
+        w[4 * a + 3] = (f1) * (f2);
+    }
+}
+
+void orc_lint3d(int na, const float *xyz, int nzpad, int nxpad, int nypad,
+                float ozpad, float oxpad, float oypad, float dz, float dx, float dy,
+                int *jz, int *jx, int *jy, float *w)
+{
+    for (int a = 0; a < na; a++) {
+        float x = xyz[3 * a], y = xyz[3 * a + 1], z = xyz[3 * a + 2];
+        float f1, f2, f3;
+        float zmax = ozpad + (float)(nzpad - 1) * dz;
+        float xmax = oxpad + (float)(nxpad - 1) * dx;
+        float ymax = oypad + (float)(nypad - 1) * dy;
+        if (z >= ozpad && z < zmax && x >= oxpad && x < xmax && y >= oypad && y < ymax) {
+            jz[a] = (int)((z - ozpad) / dz);
+            jx[a] = (int)((x - oxpad) / dx);
+            jy[a] = (int)((y - oypad) / dy);
+            f1 = (z - ozpad) / dz - (float)jz[a];
+            f2 = (x - oxpad) / dx - (float)jx[a];
+            f3 = (y - oypad) / dy - (float)jy[a];
+        } else {
+            jz[a] = 0; jx[a] = 0; jy[a] = 0; f1 = 1; f2 = 0; f3 = 0;
+        }
+        w[8 * a + 0] = (1 - f3) * (1 - f1) * (1 - f2);
+        w[8 * a + 1] = (1 - f3) * (f1) * (1 - f2);
+        w[8 * a + 2] = (1 - f3) * (1 - f1) * (f2);
+        w[8 * a + 3] = (1 - f3) * (f1) * (f2);
+        w[8 * a + 4] = (f3) * (1 - f1) * (1 - f2);
+        w[8 * a + 5] = (f3) * (f1) * (1 - f2);
+        w[8 * a + 6] = (f3) * (1 - f1) * (f2);
+        w[8 * a + 7] = (f3) * (f1) * (f2);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Spherical injection reflectivity, spher_utils.c:153-173 (2-D) and
+ * 222-310 (3-D): |v0-vk|/(v0+vk) for the first neighbour k (fixed search
+ * order) whose velocity differs from the injection node; 0 if none. */
+static float orc_refl(const float *vel, long a0, const long *offs, int noff)
+{
+    float v0 = vel[a0];
+    for (int k = 0; k < noff; k++) {
+        float vk = vel[a0 + offs[k]];
+        if (v0 != vk) return (float)fabs((double)((v0 - vk) / (v0 + vk)));
+    }
+    return 0.f;
+}
+
+static float orc_refl2d(const float *vel, int jr, int jth, int nrpad)
+{
+    /* index = th*nrpad + r ; order (z=th): z+1, x+1, z+1x+1, x-1, z-1x-1, z+1x-1, z-1x+1 */
+    long N = nrpad;
+    long offs[7] = {N, 1, N + 1, -1, -N - 1, N - 1, -N + 1};
+    return orc_refl(vel, (long)jth * N + jr, offs, 7);
+}
+
+static float orc_refl3d(const float *vel, int jr, int jph, int jth, int nrpad, int nthpad)
+{
+    long X = 1, Z = nrpad, Y = (long)nrpad * nthpad;
+    long offs[27] = {
+        Z, Y, X, -Z, -Y, -X,
+        Z + X, -Z - X, Z - X, -Z + X,
+        Y + Z, -Y - Z, Y - Z, -Y + Z,
+        Y + X, -Y - X, Y - X, -Y + X,
+        Y - X,
+        Y + Z + X, Y + Z - X, Y - Z + X, Y - Z - X,
+        -Y + Z + X, -Y + Z - X, -Y - Z + X, -Y - Z - X,
+    };
+    return orc_refl(vel, (long)jph * Y + (long)jth * Z + jr, offs, 27);
+}
+
+/* ------------------------------------------------------------------ */
+/* Spherical one-way ABC rule (spher_utils.c:583-631), serial semantics. */
+#define ORC_NOP 4
+int orc_spher3d_rule(int ix, int iz, int iy, int nxpad, int nzpad, int nypad)
+{
+    int iop, rule = 0;
+    if (ix < ORC_NOP) iop = ix;
+    else if (ix > nxpad - ORC_NOP) iop = nxpad - ix;
+    else if (iz < ORC_NOP) iop = iz;
+    else if (iz > nzpad - ORC_NOP) iop = nzpad - iz;
+    else if (iy < ORC_NOP) iop = iy;
+    else if (iy > nypad - ORC_NOP) iop = nypad - iy;
+    else iop = 0;
+    if (iz <= ORC_NOP - iop) rule = 1;
+    if (iz >= nzpad - ORC_NOP + iop - 1) rule = 2;
+    if (ix <= ORC_NOP - iop) rule = 3;
+    if (ix >= nxpad - ORC_NOP + iop - 1) rule = 4;
+    if (iy <= ORC_NOP - iop) rule = 5;
+    if (iy >= nypad - ORC_NOP + iop - 1) rule = 6;
+    return rule;
+}
+
+/* ------------------------------------------------------------------ */
+/* Propagators, instantiated for float and double. */
+#define REAL float
+#define SFX(name) name##_f32
+#define ORC_FDCOEF(nop, ndim, d, optfd, order, out) orc_fdcoef(nop, ndim, d, optfd, order, out)
+#define ORC_SNAP orc_snap_f32
+#include "oracle_prop.inc"
+#undef REAL
+#undef SFX
+#undef ORC_FDCOEF
+#undef ORC_SNAP
+
+#define REAL double
+#define SFX(name) name##_f64
+#define ORC_FDCOEF(nop, ndim, d, optfd, order, out) orc_fdcoef_d(nop, ndim, d, optfd, order, out)
+#define ORC_SNAP orc_snap_f64
+#include "oracle_prop.inc"

@@ -1,0 +1,108 @@
+/*
+ * CPU oracle for the acoustic finite-difference time loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is an independent C restatement of the
+ * reference CPU programs sfAWEFDcart2Dcpu, sfAWEFDcart3Dcpu, sfAWEFDspher2Dcpu
+ * and sfAWEFDspher3Dcpu (Nacnud04/wavefield_imaging, CODE/).  It is the checker
+ * for the HIP library: only tests, the smoke check and the CPU-baseline leg of
+ * the benchmark may load it.  The product never links, calls or falls back to it.
+ *
+ * Every routine cites the reference file:line it restates.  The FP32 entry
+ * points (suffix _f32) follow the reference's C float/double evaluation order
+ * exactly (built with -ffp-contract=off, no fast-math); the FP64 entry points
+ * (suffix _f64) run the same algorithm in double and give the FP64 error.
+ *
+ * Pinning (see tests/test_oracle.py): FD coefficients against closed forms;
+ * index map against hand-computed cases; sponge tables against the closed
+ * form; the stencils against brute-force and analytic invariants on tiny grids
+ * (constant field, impulse response symmetry, linearity, 1-D reduction); the
+ * time ordering by impulse tests (trace row 0 is zero, first arrival index).
+ */
+#ifndef AWEFD_ORACLE_H
+#define AWEFD_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Parameter block, mirroring the Madagascar parameters of the four programs.
+ * Axis 1 is the fastest axis of the velocity file:
+ *   cartesian: (z, x, y);  polar/spherical: (r, theta, phi). */
+typedef struct {
+    int   n[3];
+    float o[3];
+    float d[3];
+    int   nt;
+    float dt;
+    int   nb;        /* "nb=" as given; < 0 means absent                       */
+    int   fdorder;   /* cartesian only (default 4)                             */
+    int   optfd;     /* cartesian: optimized FD coefficients                   */
+    int   dabc;      /* cartesian: sponge (and 3-D one-way ABC)                */
+    int   fsrf;      /* free surface                                           */
+    int   expl;      /* cartesian: one wavelet for all sources;
+                        spherical: reflectivity-scaled injection               */
+    int   cden;      /* cartesian: constant density                            */
+    int   sse;       /* cartesian: 1 = SSE operation order (ts_kernel.c:78-138),
+                        0 = scalar order (ts_kernel.c:139-157)                 */
+    int   ns, nr;
+    int   snap, jsnap;
+} orc_par;
+
+/* Snapshot callback: padded field (the reference's u0 / h_po after the step),
+ * called when snap && it % jsnap == 0. */
+typedef void (*orc_snap_f32)(void *ud, int it, const float *u);
+typedef void (*orc_snap_f64)(void *ud, int it, const double *u);
+
+/* Forward modelling.  Inputs are host arrays in RSF order:
+ *   vel, den : unpadded model (den may be NULL)
+ *   sou, rec : coordinate tuples as in the sou=/rec= files
+ *              (cart2d (x,z); cart3d (x,y,z); spher2d (r,theta); spher3d (r,phi,theta))
+ *   wav      : wavelet stream read sequentially by the time loop
+ * Outputs:
+ *   dat      : nt*nr traces (row it = receivers at step it)
+ *   ufin     : padded final field (nullable)
+ * Returns 0 on success, <0 on an unsupported option or bad argument. */
+int orc_cart2d_f32(const orc_par *p, const float *vel, const float *den,
+                   const float *sou, const float *rec, const float *wav,
+                   float *dat, float *ufin, orc_snap_f32 cb, void *ud);
+int orc_cart2d_f64(const orc_par *p, const float *vel, const float *den,
+                   const float *sou, const float *rec, const float *wav,
+                   double *dat, double *ufin, orc_snap_f64 cb, void *ud);
+int orc_cart3d_f32(const orc_par *p, const float *vel, const float *den,
+                   const float *sou, const float *rec, const float *wav,
+                   float *dat, float *ufin, orc_snap_f32 cb, void *ud);
+int orc_cart3d_f64(const orc_par *p, const float *vel, const float *den,
+                   const float *sou, const float *rec, const float *wav,
+                   double *dat, double *ufin, orc_snap_f64 cb, void *ud);
+int orc_spher2d_f32(const orc_par *p, const float *vel,
+                    const float *sou, const float *rec, const float *wav,
+                    float *dat, float *ufin, orc_snap_f32 cb, void *ud);
+int orc_spher2d_f64(const orc_par *p, const float *vel,
+                    const float *sou, const float *rec, const float *wav,
+                    double *dat, double *ufin, orc_snap_f64 cb, void *ud);
+int orc_spher3d_f32(const orc_par *p, const float *vel,
+                    const float *sou, const float *rec, const float *wav,
+                    float *dat, float *ufin, orc_snap_f32 cb, void *ud);
+int orc_spher3d_f64(const orc_par *p, const float *vel,
+                    const float *sou, const float *rec, const float *wav,
+                    double *dat, double *ufin, orc_snap_f64 cb, void *ud);
+
+/* Building blocks exposed for the known-answer tests. */
+int  orc_fdcoef(int nop, int ndim, const float *d, int optfd, int order,
+                float *out /* ndim*nop+1 */);
+void orc_sponge_table(int nb, float *w /* nb */);
+void orc_lint2d(int na, const float *xz, int nzpad, int nxpad, float ozpad,
+                float oxpad, float dz, float dx, int *jz, int *jx, float *w /* 4*na */);
+void orc_lint3d(int na, const float *xyz, int nzpad, int nxpad, int nypad,
+                float ozpad, float oxpad, float oypad, float dz, float dx, float dy,
+                int *jz, int *jx, int *jy, float *w /* 8*na */);
+double orc_spher_damp(int i, int nb);
+/* ABC rule of the spherical one-way boundary for one cell (0 none, 1 th-low,
+ * 2 th-high, 3 r-low, 4 r-high, 5 ph-low, 6 ph-high): the last rule that the
+ * reference's serial sweep applies at that cell. */
+int  orc_spher3d_rule(int ir, int ith, int iph, int nrpad, int nthpad, int nphpad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,65 @@
+"""Cartesian HIP kernels vs the CPU oracle (bit-exact FP32), through the C ABI."""
+import numpy as np
+import pytest
+
+from wavefield_imaging_amd import awefd, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _case2d(nz=40, nx=56, nt=120, nb=8, **kw):
+    c = synth.cart2d(scale=0.1, nt=nt)
+    vel = synth.layered_2d(nz, nx, seed=11, air=6, base_mean=25, amp=3.0, corr=5.0)
+    sou = np.array([[nx * 5 * 0.45, 9 * 5.0]], np.float32)
+    rec = np.stack([np.arange(nx) * 5.0, np.full(nx, 7 * 5.0)], 1).astype(np.float32)
+    par = dict(n=[nz, nx], o=[0.0, 0.0], d=[5.0, 5.0], nt=nt, dt=synth.DT, nb=nb,
+               dabc=True, cden=True, fdorder=4, expl=False)
+    par.update(kw)
+    return par, vel, sou, rec, c["wav"][:nt]
+
+
+def _case3d(nz=24, nx=28, ny=30, nt=60, nb=6, **kw):
+    vel = synth.layered_3d(nz, nx, ny, seed=5, air=5, base_mean=15, amp=2.0, corr=4.0)
+    sou = np.array([[nx * 2.5 + 1.3, ny * 2.5 - 2.1, 6 * 5.0 + 0.7]], np.float32)
+    ix, iy = np.meshgrid(np.arange(0, nx, 3), np.arange(0, ny, 3), indexing="ij")
+    rec = np.stack([ix.ravel() * 5.0, iy.ravel() * 5.0, np.full(ix.size, 4 * 5.0)], 1).astype(np.float32)
+    par = dict(n=[nz, nx, ny], o=[0.0] * 3, d=[5.0] * 3, nt=nt, dt=synth.DT, nb=nb,
+               dabc=True, cden=True, fdorder=4, expl=False)
+    par.update(kw)
+    return par, vel, sou, rec, synth.ricker(nt, synth.DT)
+
+
+def _check(kind, par, vel, sou, rec, wav, orc):
+    print("case", kind, par["n"], {k: par[k] for k in par if k not in ("n", "o", "d")}, flush=True)
+    tr, _, field = awefd.run(kind, par, vel, sou, rec, wav)
+    ref, ufin, _ = orc.run(kind, par, vel, sou, rec, wav)
+    assert np.any(ref != 0)
+    np.testing.assert_array_equal(tr, ref)
+    np.testing.assert_array_equal(field, ufin)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(), dict(dabc=False), dict(fdorder=2), dict(fdorder=6), dict(fdorder=8),
+    dict(sse=False), dict(optfd=True), dict(free=True), dict(nb=None),
+])
+def test_cart2d_bit_exact(orc, kw):
+    _check("cart2d", *_case2d(**kw), orc)
+
+
+def test_cart2d_many_sources_expl(orc):
+    par, vel, sou, rec, wav = _case2d(expl=True)
+    sou = np.stack([np.arange(3, 50, 2) * 5.0 + 0.5, np.full(24, 12 * 5.0 + 1.2)], 1).astype(np.float32)
+    _check("cart2d", par, vel, sou, rec, wav, orc)
+
+
+def test_cart2d_many_sources_separate_wavelets(orc):
+    par, vel, sou, rec, wav = _case2d()
+    sou = np.array([[100.0, 60.0], [100.0, 60.0], [103.0, 61.0], [-50.0, 10.0]], np.float32)
+    w4 = np.stack([wav, -0.5 * wav, 2 * wav, wav], 1).astype(np.float32)
+    _check("cart2d", par, vel, sou, rec, w4, orc)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(dabc=False), dict(free=True), dict(fdorder=6),
+                                dict(sse=False), dict(nb=3)])
+def test_cart3d_bit_exact(orc, kw):
+    _check("cart3d", *_case3d(**kw), orc)

@@ -1,0 +1,666 @@
+/*
+ * libawefd C ABI (include/awefd.h): lifecycle, device memory, the time loop.
+ *
+ * Host C over the HIP runtime API; the kernels live in cart_kernels.hip and
+ * spher_kernels.hip and are reached through the launchers of
+ * awefd_internal.h.  Nothing here does field arithmetic on the CPU except
+ * the set-up tables (awefd_setup.c) and the pending sponge factor applied to
+ * snapshot copies.
+ */
+#define __HIP_PLATFORM_AMD__ 1
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "awefd.h"
+#include "awefd_internal.h"
+#include "awefd_model.h"
+#include "awefd_setup.h"
+
+static __thread char g_err[512];
+
+int awefd__fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+const char *awefd_last_error(void) { return g_err; }
+
+#define HIPCHK(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return awefd__fail(AWEFD_EHIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_),  \
+                               __FILE__, __LINE__);                                         \
+    } while (0)
+
+int awefd_params_default(int kind, awefd_params *p)
+{
+    if (!p || kind < AWEFD_CART2D || kind > AWEFD_SPHER3D)
+        return awefd__fail(AWEFD_EINVAL, "awefd_params_default: bad kind %d", kind);
+    memset(p, 0, sizeof *p);
+    p->kind = kind;
+    p->n[0] = p->n[1] = p->n[2] = 1;
+    p->d[0] = p->d[1] = p->d[2] = 1.f;
+    p->nb = -1;
+    p->fdorder = 4;
+    p->sse = 1;
+    p->jdata = 1;
+    if (kind == AWEFD_CART2D || kind == AWEFD_CART3D) {
+        /* MAWEFDcart2Dcpu.c:116-128: every flag defaults to n */
+        p->expl = 0;
+        p->cden = 0;
+    } else {
+        /* MAWEFDspher2Dcpu.c:56-60, MAWEFDspher3Dcpu.c:57-61: expl defaults to y */
+        p->expl = 1;
+    }
+    return AWEFD_OK;
+}
+
+static int is_cart(const awefd_model *m) { return m->p.kind <= AWEFD_CART3D; }
+
+static void free_dev(void **p)
+{
+    if (*p) hipFree(*p);
+    *p = NULL;
+}
+
+void awefd_destroy(awefd_model *m)
+{
+    if (!m) return;
+    hipSetDevice(m->dev);
+    void **ptrs[] = {(void **)&m->dP,     (void **)&m->dC,      (void **)&m->dF,
+                     (void **)&m->dvel,   (void **)&m->drho,    (void **)&m->dsp[0],
+                     (void **)&m->dsp[1], (void **)&m->dsp[2],  (void **)&m->dtab[0],
+                     (void **)&m->dtab[1], (void **)&m->dtab[2], (void **)&m->dtab[3],
+                     (void **)&m->dtab[4], (void **)&m->dtab[5], (void **)&m->d_rj,
+                     (void **)&m->d_rw,   (void **)&m->d_inj,   (void **)&m->d_ibeg,
+                     (void **)&m->d_wav,  (void **)&m->d_traces, (void **)&m->d_sj,
+                     (void **)&m->d_sw,   (void **)&m->d_srefl, (void **)&m->d_damp,
+                     (void **)&m->d_metric, (void **)&m->d_abcrules};
+    for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free_dev(ptrs[i]);
+    if (m->ev) {
+        for (int i = 0; i < m->nev; i++) hipEventDestroy(m->ev[i]);
+        free(m->ev);
+    }
+    if (m->stream) hipStreamDestroy(m->stream);
+    free(m->h_sj); free(m->h_sw); free(m->h_rj); free(m->h_rw);
+    free(m->h_sp[0]); free(m->h_sp[1]); free(m->h_sp[2]);
+    free(m->h_srefl);
+    awefd__spher_free(m);
+    free(m);
+}
+
+/* Tiling of the cartesian step kernel: z tiles of balanced width <= 64,
+ * x tiles of 16 rows, y chunks sized so that the launch has >= ~2048 blocks
+ * (8 per CU on 256 CUs). */
+static void cart_tiling(awefd_model *m)
+{
+    const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2];
+    m->ntz = (n1 + 63) / 64;
+    m->tz = (n1 + m->ntz - 1) / m->ntz;
+    m->ntx = (n2 + AWEFD_CART_TX - 1) / AWEFD_CART_TX;
+    if (m->ndim == 3) {
+        int cols = m->ntz * m->ntx;
+        int nty = (2048 + cols - 1) / cols;
+        if (nty > n3) nty = n3;
+        if (nty < 1) nty = 1;
+        m->ychunk = (n3 + nty - 1) / nty;
+        m->nty = (n3 + m->ychunk - 1) / m->ychunk;
+    } else {
+        m->ychunk = 1;
+        m->nty = 1;
+    }
+    m->nblk = m->ntz * m->ntx * m->nty;
+}
+
+int awefd_create(const awefd_params *p, awefd_model **out)
+{
+    if (!p || !out) return awefd__fail(AWEFD_EINVAL, "awefd_create: NULL argument");
+    *out = NULL;
+    if (p->kind < AWEFD_CART2D || p->kind > AWEFD_SPHER3D)
+        return awefd__fail(AWEFD_EINVAL, "awefd_create: bad kind %d", p->kind);
+    awefd_model *m = calloc(1, sizeof *m);
+    if (!m) return awefd__fail(AWEFD_ENOMEM, "awefd_create: out of host memory");
+    m->p = *p;
+    m->ndim = (p->kind == AWEFD_CART3D || p->kind == AWEFD_SPHER3D) ? 3 : 2;
+    if (m->ndim == 2) {
+        m->p.n[2] = 1;
+        m->p.d[2] = 1.f;
+        m->p.o[2] = 0.f;
+    }
+    for (int a = 0; a < m->ndim; a++)
+        if (p->n[a] < 1 || !(p->d[a] > 0.f)) {
+            free(m);
+            return awefd__fail(AWEFD_EINVAL, "awefd_create: axis %d has n=%d d=%g", a + 1,
+                               p->n[a], (double)p->d[a]);
+        }
+    if (p->nt < 1) {
+        free(m);
+        return awefd__fail(AWEFD_EINVAL, "awefd_create: nt=%d", p->nt);
+    }
+    int rc;
+    if (is_cart(m)) {
+        const int fdorder = p->fdorder > 0 ? p->fdorder : 4;
+        m->nop = fdorder / 2;
+        if (fdorder % 2 || m->nop < 1 || m->nop > AWEFD_SETUP_MAX_NOP) {
+            free(m);
+            return awefd__fail(AWEFD_EINVAL, "fdorder=%d unsupported (even, 2..12)", fdorder);
+        }
+        /* MAWEFDcart2Dcpu.c:211: nb < nop or absent -> nop */
+        m->nb = p->nb < m->nop ? m->nop : p->nb;
+    } else {
+        m->nop = 4; /* NOP of spher_utils.c:326 */
+        m->nb = p->nb < 4 ? 4 : p->nb;
+    }
+    m->ncell = 1;
+    for (int a = 0; a < 3; a++) {
+        const int nba = a < m->ndim ? m->nb : 0;
+        m->np[a] = m->p.n[a] + 2 * nba;
+        /* fdutil.c:293-297: padded origin o - nb*d in float */
+        m->opad[a] = m->p.o[a] - (float)nba * m->p.d[a];
+        m->ncell *= m->np[a];
+    }
+    if (m->ncell > (1L << 33)) {
+        free(m);
+        return awefd__fail(AWEFD_EINVAL, "grid of %ld nodes too large", m->ncell);
+    }
+    m->dev = p->device;
+    hipError_t e = hipSetDevice(m->dev);
+    if (e != hipSuccess) {
+        free(m);
+        return awefd__fail(AWEFD_EHIP, "hipSetDevice(%d): %s", p->device, hipGetErrorString(e));
+    }
+    e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        free(m);
+        return awefd__fail(AWEFD_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    if (is_cart(m)) {
+        const float d[3] = {m->p.d[0], m->p.d[1], m->p.d[2]};
+        awefd_fdcoef(m->nop, m->ndim, d, p->optfd, 2, m->c2);
+        awefd_fdcoef(m->nop, m->ndim, d, p->optfd, 1, m->c1);
+        cart_tiling(m);
+        if (m->ndim == 3 && p->dabc && (m->np[0] < 2 * m->nop + 4 || m->np[1] < 2 * m->nop + 4
+                                        || m->np[2] < 2 * m->nop + 4)) {
+            awefd_destroy(m);
+            return awefd__fail(AWEFD_EINVAL, "3-D one-way ABC needs >= %d padded nodes per axis",
+                               2 * m->nop + 4);
+        }
+        if (p->cden == 0 && 0) {
+            /* variable density handled when a density is supplied */
+        }
+    }
+    const size_t bytes = sizeof(float) * (size_t)m->ncell;
+    if ((e = hipMalloc((void **)&m->dP, bytes)) != hipSuccess
+        || (e = hipMalloc((void **)&m->dC, bytes)) != hipSuccess
+        || (e = hipMalloc((void **)&m->dvel, bytes)) != hipSuccess) {
+        awefd_destroy(m);
+        return awefd__fail(AWEFD_ENOMEM, "hipMalloc of 3 x %zu bytes: %s", bytes,
+                           hipGetErrorString(e));
+    }
+    if (!is_cart(m) && (rc = awefd__spher_create(m)) != 0) {
+        awefd_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return AWEFD_OK;
+}
+
+int awefd_padded_dims(const awefd_model *m, int dims[3], int *nb)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    if (dims)
+        for (int a = 0; a < 3; a++) dims[a] = m->np[a];
+    if (nb) *nb = m->nb;
+    return AWEFD_OK;
+}
+
+int awefd_device(const awefd_model *m) { return m ? m->dev : -1; }
+
+static int upload(awefd_model *m, void **dst, const void *src, size_t bytes)
+{
+    free_dev(dst);
+    if (bytes == 0) return AWEFD_OK;
+    HIPCHK(hipMalloc(dst, bytes));
+    HIPCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    (void)m;
+    return AWEFD_OK;
+}
+
+/* Cartesian model set-up: MAWEFDcart2Dcpu.c:258-295, MAWEFDcart3Dcpu.c:254-293. */
+static int cart_set_model(awefd_model *m, const float *vel, const float *den)
+{
+    const int nbs[3] = {m->nb, m->nb, m->ndim == 3 ? m->nb : 0};
+    float *v = malloc(sizeof(float) * m->ncell);
+    if (!v) return awefd__fail(AWEFD_ENOMEM, "host pad buffer");
+    awefd_pad(vel, v, m->p.n, nbs);
+    const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2];
+    int rc = AWEFD_OK;
+    if (m->p.dabc && m->ndim == 3) {
+        /* abcone3d_make(nb, dt, vel, ...) on the raw velocity, MAWEFDcart3Dcpu.c:270 */
+        const long sx = n1, sy = (long)n1 * n2;
+        const int nb = m->nb;
+        float *t[6];
+        size_t sz[6] = {(size_t)n2 * n3, (size_t)n2 * n3, (size_t)n1 * n3,
+                        (size_t)n1 * n3, (size_t)n1 * n2, (size_t)n1 * n2};
+        for (int i = 0; i < 6; i++) t[i] = malloc(sizeof(float) * sz[i]);
+        for (int iy = 0; iy < n3; iy++)
+            for (int ix = 0; ix < n2; ix++) {
+                t[0][iy * n2 + ix] = awefd_oneway_cart(v[iy * sy + ix * sx + nb], m->p.dt, m->p.d[0]);
+                t[1][iy * n2 + ix] = awefd_oneway_cart(v[iy * sy + ix * sx + n1 - nb - 1], m->p.dt, m->p.d[0]);
+            }
+        for (int iy = 0; iy < n3; iy++)
+            for (int iz = 0; iz < n1; iz++) {
+                t[2][iy * n1 + iz] = awefd_oneway_cart(v[iy * sy + nb * sx + iz], m->p.dt, m->p.d[1]);
+                t[3][iy * n1 + iz] = awefd_oneway_cart(v[iy * sy + (n2 - nb - 1) * sx + iz], m->p.dt, m->p.d[1]);
+            }
+        for (int ix = 0; ix < n2; ix++)
+            for (int iz = 0; iz < n1; iz++) {
+                t[4][ix * n1 + iz] = awefd_oneway_cart(v[(long)nb * sy + ix * sx + iz], m->p.dt, m->p.d[2]);
+                t[5][ix * n1 + iz] = awefd_oneway_cart(v[(long)(n3 - nb - 1) * sy + ix * sx + iz], m->p.dt, m->p.d[2]);
+            }
+        for (int i = 0; i < 6 && rc == 0; i++)
+            rc = upload(m, (void **)&m->dtab[i], t[i], sizeof(float) * sz[i]);
+        for (int i = 0; i < 6; i++) free(t[i]);
+    }
+    if (m->p.dabc && rc == 0) {
+        float w[4096];
+        if (m->nb > 4096) {
+            free(v);
+            return awefd__fail(AWEFD_EINVAL, "nb=%d too large", m->nb);
+        }
+        awefd_sponge_weights(m->nb, w);
+        for (int a = 0; a < 3 && rc == 0; a++) {
+            free(m->h_sp[a]);
+            m->h_sp[a] = malloc(sizeof(float) * m->np[a]);
+            if (a < m->ndim) awefd_sponge_axis(w, m->nb, m->np[a], m->h_sp[a]);
+            else m->h_sp[a][0] = 1.f;
+            rc = upload(m, (void **)&m->dsp[a], m->h_sp[a], sizeof(float) * m->np[a]);
+        }
+    }
+    /* vel <- vel * (vel * dt^2) ; free surface zeroes z in [0, nb] */
+    const float dt2 = m->p.dt * m->p.dt;
+    for (long i = 0; i < m->ncell; i++) v[i] *= v[i] * dt2;
+    if (m->p.fsrf)
+        for (long c = 0; c < (long)n2 * n3; c++)
+            for (int iz = 0; iz <= m->nb && iz < n1; iz++) v[c * n1 + iz] = 0.f;
+    if (rc == 0) {
+        hipError_t e = hipMemcpy(m->dvel, v, sizeof(float) * m->ncell, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "vel upload: %s", hipGetErrorString(e));
+    }
+    free(v);
+    if (rc == 0 && den && !m->p.cden)
+        rc = awefd__fail(AWEFD_EINVAL, "variable density (cden=n with den=) is not built yet");
+    return rc;
+}
+
+int awefd_set_model(awefd_model *m, const float *vel, const float *den)
+{
+    if (!m || !vel) return awefd__fail(AWEFD_EINVAL, "awefd_set_model: NULL argument");
+    HIPCHK(hipSetDevice(m->dev));
+    int rc = is_cart(m) ? cart_set_model(m, vel, den) : awefd__spher_set_model(m, vel);
+    if (rc == 0) m->model_set = 1;
+    return rc;
+}
+
+static int set_points(awefd_model *m, int which, int n, const float *coords)
+{
+    if (!m || n < 0 || (n > 0 && !coords)) return awefd__fail(AWEFD_EINVAL, "set_points: bad argument");
+    int *j = malloc(sizeof(int) * 3 * (n > 0 ? n : 1));
+    float *w = malloc(sizeof(float) * 8 * (n > 0 ? n : 1));
+    /* grid in (z, x, y) order; spherical: z = theta, x = r, y = phi */
+    int npad[3];
+    float opad[3], d[3];
+    if (is_cart(m)) {
+        for (int a = 0; a < 3; a++) {
+            npad[a] = m->np[a];
+            opad[a] = m->opad[a];
+            d[a] = m->p.d[a];
+        }
+    } else {
+        const int perm[3] = {1, 0, 2};
+        for (int a = 0; a < 3; a++) {
+            npad[a] = m->np[perm[a]];
+            opad[a] = m->opad[perm[a]];
+            d[a] = m->p.d[perm[a]];
+        }
+    }
+    awefd_lint(m->ndim, n, coords, npad, opad, d, j, w);
+    /* every interpolation corner must be a padded node: the kernels index
+     * with these values unchecked */
+    for (int a = 0; a < n; a++)
+        for (int ax = 0; ax < m->ndim; ax++)
+            if (j[3 * a + ax] < 0 || j[3 * a + ax] + 1 >= npad[ax]) {
+                free(j);
+                free(w);
+                return awefd__fail(AWEFD_EINVAL, "%s %d maps outside the padded grid",
+                                   which == 0 ? "source" : "receiver", a);
+            }
+    if (which == 0) {
+        free(m->h_sj); free(m->h_sw);
+        m->h_sj = j; m->h_sw = w; m->ns = n;
+    } else {
+        free(m->h_rj); free(m->h_rw);
+        m->h_rj = j; m->h_rw = w; m->nr = n;
+    }
+    return AWEFD_OK;
+}
+
+/* Injection buckets of the cartesian step kernel (one per z-tile/x-tile
+ * column), holding every (node, source, weight) contribution. */
+static int cart_build_inj(awefd_model *m)
+{
+    const int nc = m->ndim == 3 ? 8 : 4;
+    const long n = (long)m->ns * nc;
+    const int nbucket = m->ntz * m->ntx;
+    awefd_inj_key *keys = malloc(sizeof(*keys) * (n > 0 ? n : 1));
+    awefd_inj *e = malloc(sizeof(*e) * (n > 0 ? n : 1));
+    int *ibeg = malloc(sizeof(int) * (nbucket + 1));
+    for (int s = 0; s < m->ns; s++)
+        for (int q = 0; q < nc; q++) {
+            awefd_inj_key *k = &keys[(long)s * nc + q];
+            k->e.z = m->h_sj[3 * s] + (q & 1);
+            k->e.x = m->h_sj[3 * s + 1] + ((q >> 1) & 1);
+            k->e.y = m->ndim == 3 ? m->h_sj[3 * s + 2] + ((q >> 2) & 1) : 0;
+            k->e.src = s;
+            k->e.w = m->h_sw[8 * s + q];
+            k->bucket = (k->e.z / m->tz) + m->ntz * (k->e.x / AWEFD_CART_TX);
+        }
+    awefd_sort_inj(keys, n, nbucket, e, ibeg);
+    int rc = upload(m, (void **)&m->d_inj, e, sizeof(*e) * (n > 0 ? n : 1));
+    if (!rc) rc = upload(m, (void **)&m->d_ibeg, ibeg, sizeof(int) * (nbucket + 1));
+    m->ninj = n;
+    free(keys); free(e); free(ibeg);
+    return rc;
+}
+
+int awefd_set_sources(awefd_model *m, int ns, const float *coords)
+{
+    int rc = set_points(m, 0, ns, coords);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(m->dev));
+    rc = is_cart(m) ? cart_build_inj(m) : awefd__spher_set_sources(m);
+    if (!rc) m->src_set = 1;
+    return rc;
+}
+
+int awefd_set_receivers(awefd_model *m, int nr, const float *coords)
+{
+    int rc = set_points(m, 1, nr, coords);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(m->dev));
+    rc = upload(m, (void **)&m->d_rj, m->h_rj, sizeof(int) * 3 * (nr > 0 ? nr : 1));
+    if (!rc) rc = upload(m, (void **)&m->d_rw, m->h_rw, sizeof(float) * 8 * (nr > 0 ? nr : 1));
+    if (!rc) m->rec_set = 1;
+    return rc;
+}
+
+int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav)
+{
+    if (!m || nsamples < 1 || !wav) return awefd__fail(AWEFD_EINVAL, "awefd_set_wavelet: bad argument");
+    HIPCHK(hipSetDevice(m->dev));
+    int rc = upload(m, (void **)&m->d_wav, wav, sizeof(float) * nsamples);
+    if (!rc) {
+        m->nwav = nsamples;
+        m->wav_set = 1;
+    }
+    return rc;
+}
+
+int awefd_debug_points(const awefd_model *m, int which, int *idx, float *w)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    const int n = which == 0 ? m->ns : m->nr;
+    const int *j = which == 0 ? m->h_sj : m->h_rj;
+    const float *ww = which == 0 ? m->h_sw : m->h_rw;
+    if (!j) return awefd__fail(AWEFD_ESTATE, "points not set");
+    /* report (fastest, middle, slowest) axis indices of the model */
+    for (int a = 0; a < n; a++) {
+        if (is_cart(m)) {
+            idx[3 * a] = j[3 * a];
+            idx[3 * a + 1] = j[3 * a + 1];
+            idx[3 * a + 2] = j[3 * a + 2];
+        } else { /* spherical: (r, theta, phi) from the (theta, r, phi) map */
+            idx[3 * a] = j[3 * a + 1];
+            idx[3 * a + 1] = j[3 * a];
+            idx[3 * a + 2] = j[3 * a + 2];
+        }
+        memcpy(w + 8 * a, ww + 8 * a, 8 * sizeof(float));
+    }
+    return AWEFD_OK;
+}
+
+int awefd_reset(awefd_model *m)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    HIPCHK(hipSetDevice(m->dev));
+    int e = awefd_launch_fill(m->dP, m->ncell, 0.f, m->stream);
+    if (!e) e = awefd_launch_fill(m->dC, m->ncell, 0.f, m->stream);
+    if (!e && m->dF) e = awefd_launch_fill(m->dF, m->ncell, 0.f, m->stream);
+    if (e) return awefd__fail(AWEFD_EHIP, "reset: %s", hipGetErrorString((hipError_t)e));
+    m->steps_done = 0;
+    return AWEFD_OK;
+}
+
+int awefd_profile(awefd_model *m, int enable)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    m->prof = enable;
+    m->prof_launches = 0;
+    m->prof_ms = 0.0;
+    return AWEFD_OK;
+}
+
+int awefd_kernel_stats(awefd_model *m, long *launches, double *total_ms)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    if (launches) *launches = m->prof_launches;
+    if (total_ms) *total_ms = m->prof_ms;
+    return AWEFD_OK;
+}
+
+/* Events around the dominant kernel: pairs (start, stop) per launch, read
+ * back after the loop. */
+static int prof_reserve(awefd_model *m, int n)
+{
+    if (m->nev >= n) return AWEFD_OK;
+    hipEvent_t *ev = realloc(m->ev, sizeof(hipEvent_t) * n);
+    if (!ev) return awefd__fail(AWEFD_ENOMEM, "event array");
+    m->ev = ev;
+    for (int i = m->nev; i < n; i++) HIPCHK(hipEventCreate(&m->ev[i]));
+    m->nev = n;
+    return AWEFD_OK;
+}
+
+static int prof_collect(awefd_model *m, int npairs)
+{
+    HIPCHK(hipEventSynchronize(m->ev[2 * npairs - 1]));
+    for (int i = 0; i < npairs; i++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, m->ev[2 * i], m->ev[2 * i + 1]));
+        m->prof_ms += ms;
+    }
+    m->prof_launches += npairs;
+    return AWEFD_OK;
+}
+
+/* Apply the pending sponge factor to a host copy of the stored field (the
+ * reference's u0 view), in the reference's z, x, y order. */
+static void cart_apply_pending(const awefd_model *m, float *u)
+{
+    if (!m->p.dabc) return;
+    const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2], nb = m->nb;
+    for (int iy = 0; iy < n3; iy++)
+        for (int ix = 0; ix < n2; ix++)
+            for (int iz = 0; iz < n1; iz++) {
+                const int s = iz < nb || iz >= n1 - nb || ix < nb || ix >= n2 - nb
+                              || (m->ndim == 3 && (iy < nb || iy >= n3 - nb));
+                if (!s) continue;
+                float *p = &u[((long)iy * n2 + ix) * n1 + iz];
+                float v = *p * m->h_sp[0][iz];
+                v = v * m->h_sp[1][ix];
+                if (m->ndim == 3) v = v * m->h_sp[2][iy];
+                *p = v;
+            }
+}
+
+int awefd_get_field(awefd_model *m, float *host)
+{
+    if (!m || !host) return awefd__fail(AWEFD_EINVAL, "awefd_get_field: NULL argument");
+    HIPCHK(hipSetDevice(m->dev));
+    if (!is_cart(m)) return awefd__spher_get_field(m, host);
+    HIPCHK(hipStreamSynchronize(m->stream));
+    HIPCHK(hipMemcpy(host, m->dP, sizeof(float) * m->ncell, hipMemcpyDeviceToHost));
+    cart_apply_pending(m, host);
+    return AWEFD_OK;
+}
+
+static void cart_args(const awefd_model *m, awefd_cart_args *a)
+{
+    memset(a, 0, sizeof *a);
+    a->vel = m->dvel;
+    a->rho = m->drho;
+    a->n1 = m->np[0];
+    a->n2 = m->np[1];
+    a->n3 = m->np[2];
+    const int nop = m->nop;
+    a->c0 = m->c2[0];
+    for (int k = 1; k <= nop; k++) {
+        a->cz[k - 1] = m->c2[k];
+        a->cx[k - 1] = m->c2[nop + k];
+        a->cy[k - 1] = m->ndim == 3 ? m->c2[2 * nop + k] : 0.f;
+        a->bz[k - 1] = m->c1[k];
+        a->bx[k - 1] = m->c1[nop + k];
+        a->by[k - 1] = m->ndim == 3 ? m->c1[2 * nop + k] : 0.f;
+    }
+    a->tz = m->tz;
+    a->ntz = m->ntz;
+    a->ntx = m->ntx;
+    a->ychunk = m->ychunk;
+    a->nty = m->nty;
+    a->nblk = m->nblk;
+    a->dabc = m->p.dabc;
+    a->nb = m->nb;
+    a->sp1 = m->dsp[0];
+    a->sp2 = m->dsp[1];
+    a->sp3 = m->dsp[2];
+    a->abc = m->p.dabc && m->ndim == 3;
+    a->fsrf = m->p.fsrf;
+    a->bzl = m->dtab[0];
+    a->bzh = m->dtab[1];
+    a->bxl = m->dtab[2];
+    a->bxh = m->dtab[3];
+    a->byl = m->dtab[4];
+    a->byh = m->dtab[5];
+    a->ibeg = m->d_ibeg;
+    a->inj = m->d_inj;
+    a->wav = m->d_wav;
+    a->wexpl = m->p.expl;
+    a->wstride = m->p.expl ? 1 : m->ns;
+    a->nr = m->nr;
+    a->rj = m->d_rj;
+    a->rw = m->d_rw;
+}
+
+static int check_ready(awefd_model *m)
+{
+    if (!m->model_set) return awefd__fail(AWEFD_ESTATE, "velocity not set");
+    if (!m->src_set) return awefd__fail(AWEFD_ESTATE, "sources not set");
+    if (!m->rec_set) return awefd__fail(AWEFD_ESTATE, "receivers not set");
+    if (!m->wav_set) return awefd__fail(AWEFD_ESTATE, "wavelet not set");
+    long need;
+    if (is_cart(m)) need = (long)m->p.nt * (m->p.expl ? 1 : m->ns);
+    else need = m->p.nt;
+    if (m->nwav < need)
+        return awefd__fail(AWEFD_EINVAL, "wavelet has %ld samples, the loop reads %ld", m->nwav, need);
+    return AWEFD_OK;
+}
+
+static int ensure_traces(awefd_model *m)
+{
+    const long n = (long)m->p.nt * m->nr;
+    if (m->ntraces >= n && m->d_traces) return AWEFD_OK;
+    free_dev((void **)&m->d_traces);
+    HIPCHK(hipMalloc((void **)&m->d_traces, sizeof(float) * (n > 0 ? n : 1)));
+    m->ntraces = n;
+    return AWEFD_OK;
+}
+
+/* The cartesian time loop on the device. */
+static int cart_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, void *user)
+{
+    awefd_cart_args a;
+    cart_args(m, &a);
+    const int nt = m->p.nt;
+    float *P = m->dP, *C = m->dC;
+    float *host = NULL;
+    int npairs = 0, rc = AWEFD_OK;
+    if (m->prof && (rc = prof_reserve(m, 2 * nt))) return rc;
+    for (int it = 0; it < nt; it++) {
+        a.P = P;
+        a.C = C;
+        a.it = it;
+        a.trow = dtr ? dtr + (long)it * m->nr : m->d_traces + (long)it * m->nr;
+        if (m->prof) HIPCHK(hipEventRecord(m->ev[2 * npairs], m->stream));
+        int e = awefd_launch_cart_step(&a, m->nop, m->ndim, m->p.sse, 0, m->stream);
+        if (m->prof) HIPCHK(hipEventRecord(m->ev[2 * npairs + 1], m->stream));
+        if (m->prof) npairs++;
+        if (!e && a.abc) e = awefd_launch_cart_abc_xy(&a, m->nop, m->stream);
+        if (e) return awefd__fail(AWEFD_EHIP, "step %d launch: %s", it, hipGetErrorString((hipError_t)e));
+        float *t = P;
+        P = C;
+        C = t;
+        m->dP = P;
+        m->dC = C;
+        m->steps_done = it + 1;
+        if (cb && jsnap > 0 && it % jsnap == 0) {
+            if (!host && !(host = malloc(sizeof(float) * m->ncell))) return awefd__fail(AWEFD_ENOMEM, "snapshot buffer");
+            HIPCHK(hipStreamSynchronize(m->stream));
+            HIPCHK(hipMemcpy(host, P, sizeof(float) * m->ncell, hipMemcpyDeviceToHost));
+            cart_apply_pending(m, host);
+            cb(user, it, host, m->np);
+        }
+    }
+    free(host);
+    if (m->prof && npairs) rc = prof_collect(m, npairs);
+    return rc;
+}
+
+static int run_common(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, void *user)
+{
+    int rc = check_ready(m);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(m->dev));
+    if ((rc = ensure_traces(m))) return rc;
+    if ((rc = awefd_reset(m))) return rc;
+    if (is_cart(m)) rc = cart_loop(m, dtr, jsnap, cb, user);
+    else rc = awefd__spher_loop(m, dtr, jsnap, cb, user);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(m->stream));
+    return AWEFD_OK;
+}
+
+int awefd_run(awefd_model *m, float *traces, int jsnap, awefd_snap_fn cb, void *user)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    int rc = run_common(m, NULL, jsnap, cb, user);
+    if (rc) return rc;
+    if (traces && m->nr > 0)
+        HIPCHK(hipMemcpy(traces, m->d_traces, sizeof(float) * (size_t)m->p.nt * m->nr,
+                         hipMemcpyDeviceToHost));
+    return AWEFD_OK;
+}
+
+int awefd_run_dev(awefd_model *m, float *d_traces)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    return run_common(m, d_traces, 0, NULL, NULL);
+}

@@ -1,0 +1,75 @@
+/* Private state of an awefd_model handle (host C). */
+#ifndef AWEFD_MODEL_H
+#define AWEFD_MODEL_H
+
+#define __HIP_PLATFORM_AMD__ 1
+#include <hip/hip_runtime_api.h>
+
+#include "awefd.h"
+#include "awefd_internal.h"
+
+struct awefd_model {
+    awefd_params p;
+    int ndim;
+    int nop;            /* stencil half-width (cartesian) / NOP=4 (spherical) */
+    int nb;             /* effective padding                                 */
+    int np[3];          /* padded dims, fastest first (model axis order)     */
+    float opad[3];      /* padded origins                                    */
+    long ncell;
+    int model_set, src_set, rec_set, wav_set;
+    float c2[3 * AWEFD_MAX_NOP + 1], c1[3 * AWEFD_MAX_NOP + 1];
+
+    int dev;
+    hipStream_t stream;
+    float *dP, *dC, *dF;          /* fields (dF: third field, spherical)     */
+    float *dvel, *drho;
+    float *dsp[3];                /* cartesian sponge factors per axis       */
+    float *h_sp[3];
+    float *dtab[6];               /* one-way ABC tables                      */
+
+    /* sources / receivers: index map (3 ints, grid z/x/y order) and weights */
+    int ns, nr;
+    int *h_sj, *h_rj;
+    float *h_sw, *h_rw;
+    int *d_sj, *d_rj;
+    float *d_sw, *d_rw;
+    float *h_srefl, *d_srefl;     /* spherical reflectivity scale per source */
+    awefd_inj *d_inj;
+    int *d_ibeg;
+    long ninj;
+
+    float *d_wav;
+    long nwav;
+    float *d_traces;
+    long ntraces;
+
+    /* cartesian tiling */
+    int tz, ntz, ntx, ychunk, nty, nblk;
+
+    /* spherical state (spher_host.c) */
+    double *d_damp;
+    void *d_metric;
+    void *d_abcrules;
+    void *spher;
+
+    /* profiling */
+    int prof;
+    long prof_launches;
+    double prof_ms;
+    hipEvent_t *ev;
+    int nev;
+
+    int steps_done;
+};
+
+int awefd__fail(int code, const char *fmt, ...);
+
+/* spherical back end (spher_host.c) */
+int  awefd__spher_create(awefd_model *m);
+int  awefd__spher_set_model(awefd_model *m, const float *vel);
+int  awefd__spher_set_sources(awefd_model *m);
+int  awefd__spher_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, void *user);
+int  awefd__spher_get_field(awefd_model *m, float *host);
+void awefd__spher_free(awefd_model *m);
+
+#endif

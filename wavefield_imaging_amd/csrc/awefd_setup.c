@@ -1,0 +1,232 @@
+/*
+ * Host-side set-up arithmetic of libawefd: finite-difference coefficients,
+ * padding, index maps, sponge and one-way coefficient tables, injection
+ * buckets.  Everything here must round exactly like the reference's C code,
+ * so this file is compiled by gcc with -ffp-contract=off and no fast-math,
+ * and every expression keeps the reference's float/double types.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "awefd_setup.h"
+
+#define SETUP_PI 3.14159265358979323846264338328
+
+/* MAWEFDcart2Dcpu.c:476-484 */
+static int fact(int n)
+{
+    int r = 1;
+    for (int i = 2; i <= n; i++) r *= i;
+    return r;
+}
+
+/* Optimized coefficients of optfd=y (Liu 2013 second derivative; Zhang & Yao
+ * 2013 first derivative), the values the reference tabulates at
+ * MAWEFDcart2Dcpu.c:508-549, rounded to float as it stores them. */
+static const float OPT_D2[7][7] = {
+    {0},
+    {0.f, 1.f},
+    {0.f, 1.369074f, -(float)0.09266816},
+    {0.f, 1.573661f, -0.1820268f, 0.01728053f},
+    {0.f, 1.700010f, -0.2554615f, 0.04445392f, -0.004946851f},
+    {0.f, 1.782836f, -0.3124513f, 0.07379487f, -0.01532122f, 0.001954439f},
+    {0.f, 1.837023f, -0.3538895f, 0.09978343f, -0.02815486f, 0.006556587f, -0.0009405699f},
+};
+static const float OPT_D1[7][7] = {
+    {0},
+    {0.f, 0.5f},
+    {0.f, (float)0.67880327, -(float)0.08962729},
+    {0.f, (float)0.77793115, -(float)0.17388691, (float)0.02338713},
+    {0.f, (float)0.84149635, -(float)0.24532989, (float)0.06081891, -(float)0.00839807},
+    {0.f, (float)0.88414717, -(float)0.30233648, (float)0.10275057, -(float)0.02681517,
+     (float)0.00398089},
+    {0.f, (float)0.91067892, -(float)0.34187892, (float)0.13833962, -(float)0.04880710,
+     (float)0.01302148, -(float)0.00199047},
+};
+
+/* Binomial-window weights (Chu & Stoffa 2012), MAWEFDcart2Dcpu.c:486-500:
+ * c_n = -2/n^2 cos(n pi) (nop!)^2 / ((nop+n)! (nop-n)!); the first
+ * derivative weight is c_n n / 2.  Types as the reference: the float
+ * quotient -2.f/(n*n) times the double cosine, rounded to float. */
+static void binomial_weights(int nop, int order, float *c)
+{
+    int hf = fact(nop);
+    hf *= hf;
+    c[0] = 0.f;
+    for (int n = 1; n <= nop; n++) {
+        const float lead = -2.f / (float)(n * n);
+        c[n] = (float)((double)lead * cos(n * SETUP_PI) * (double)hf / (double)fact(nop + n)
+                       / (double)fact(nop - n));
+        if (order == 1) c[n] = c[n] * (float)n / 2.f;
+    }
+}
+
+int awefd_fdcoef(int nop, int ndim, const float *d, int optfd, int order, float *out)
+{
+    float c[AWEFD_SETUP_MAX_NOP + 1];
+    if (nop < 1 || nop > AWEFD_SETUP_MAX_NOP || ndim < 2 || ndim > 3) return -1;
+    if (optfd) memcpy(c, order == 2 ? OPT_D2[nop] : OPT_D1[nop], sizeof(float) * (nop + 1));
+    else binomial_weights(nop, order, c);
+    out[0] = 0.f;
+    for (int a = 0; a < ndim; a++) {
+        const float h = order == 2 ? d[a] * d[a] : d[a];
+        for (int k = 1; k <= nop; k++) {
+            out[a * nop + k] = c[k] / h;
+            if (order == 2) out[0] += out[a * nop + k];
+        }
+    }
+    if (order == 2) out[0] *= -2.0f;
+    return 0;
+}
+
+/* fdutil.c:2181-2203 */
+void awefd_sponge_weights(int nb, float *w)
+{
+    const float sb = (float)(4.0 * nb);
+    for (int ib = 0; ib < nb; ib++) {
+        const float fb = (float)(ib / (sqrt(2.0) * (double)sb));
+        const float arg = -fb * fb;
+        w[ib] = (float)exp((double)arg);
+    }
+}
+
+/* Per-axis sponge factor of padded node i on an axis of n nodes: the strip
+ * value w[nb-1-ib] for the ib-th node from either face, 1 elsewhere
+ * (fdutil.c:2206-2246). */
+void awefd_sponge_axis(const float *w, int nb, int n, float *f)
+{
+    for (int i = 0; i < n; i++) f[i] = 1.0f;
+    for (int ib = 0; ib < nb && ib < n; ib++) {
+        f[ib] = w[nb - ib - 1];
+        f[n - ib - 1] = w[nb - ib - 1];
+    }
+}
+
+/* Spherical sponge factor, spher_utils.c:665-667 (double). */
+double awefd_spher_damp(int i, int nb)
+{
+    const float alpha = 0.90f;
+    return exp(-1.0 * fabs((pow((i - 1.0), 2) * log((double)alpha)) / (pow(nb, 2))));
+}
+
+/* Edge-replicating pad (expand2d/expand3d, fdutil.c:435-571; expand_cpu_2d/3d,
+ * spher_utils.c:13-103): padded node (i1,i2,i3) takes model node
+ * (clamp(i1-nb1), clamp(i2-nb2), clamp(i3-nb3)). */
+void awefd_pad(const float *a, float *b, const int n[3], const int nbs[3])
+{
+    const long p1 = n[0] + 2L * nbs[0], p2 = n[1] + 2L * nbs[1], p3 = n[2] + 2L * nbs[2];
+    for (long i3 = 0; i3 < p3; i3++) {
+        long k3 = i3 - nbs[2];
+        k3 = k3 < 0 ? 0 : (k3 >= n[2] ? n[2] - 1 : k3);
+        for (long i2 = 0; i2 < p2; i2++) {
+            long k2 = i2 - nbs[1];
+            k2 = k2 < 0 ? 0 : (k2 >= n[1] ? n[1] - 1 : k2);
+            const float *src = a + (k3 * n[1] + k2) * n[0];
+            float *dst = b + (i3 * p2 + i2) * p1;
+            for (long i1 = 0; i1 < p1; i1++) {
+                long k1 = i1 - nbs[0];
+                k1 = k1 < 0 ? 0 : (k1 >= n[0] ? n[0] - 1 : k1);
+                dst[i1] = src[k1];
+            }
+        }
+    }
+}
+
+/* lint2d_make / lint3d_make (fdutil.c:1363-1411, 1474-1539), generalised to
+ * ndim axes.  coords holds (x[,y],z) tuples as the sou=/rec= files do; the
+ * grid arguments are (z, x, y).  Float arithmetic and truncation exactly as
+ * the reference; out-of-grid points map to node 0 with the weight on z+1. */
+void awefd_lint(int ndim, int na, const float *coords, const int npad[3], const float opad[3],
+                const float d[3], int *j /* 3*na */, float *w /* 8*na */)
+{
+    for (int a = 0; a < na; a++) {
+        float x, y = 0.f, z;
+        if (ndim == 2) {
+            x = coords[2 * a];
+            z = coords[2 * a + 1];
+        } else {
+            x = coords[3 * a];
+            y = coords[3 * a + 1];
+            z = coords[3 * a + 2];
+        }
+        const float zmax = opad[0] + (float)(npad[0] - 1) * d[0];
+        const float xmax = opad[1] + (float)(npad[1] - 1) * d[1];
+        int inside = z >= opad[0] && z < zmax && x >= opad[1] && x < xmax;
+        if (ndim == 3) {
+            const float ymax = opad[2] + (float)(npad[2] - 1) * d[2];
+            inside = inside && y >= opad[2] && y < ymax;
+        }
+        float f1 = 1.f, f2 = 0.f, f3 = 0.f;
+        int jz = 0, jx = 0, jy = 0;
+        if (inside) {
+            jz = (int)((z - opad[0]) / d[0]);
+            jx = (int)((x - opad[1]) / d[1]);
+            f1 = (z - opad[0]) / d[0] - (float)jz;
+            f2 = (x - opad[1]) / d[1] - (float)jx;
+            if (ndim == 3) {
+                jy = (int)((y - opad[2]) / d[2]);
+                f3 = (y - opad[2]) / d[2] - (float)jy;
+            }
+        }
+        j[3 * a] = jz;
+        j[3 * a + 1] = jx;
+        j[3 * a + 2] = jy;
+        float *wa = w + 8 * a;
+        if (ndim == 2) {
+            wa[0] = (1 - f1) * (1 - f2);
+            wa[1] = (f1) * (1 - f2);
+            wa[2] = (1 - f1) * (f2);
+            wa[3] = (f1) * (f2);
+            wa[4] = wa[5] = wa[6] = wa[7] = 0.f;
+        } else {
+            wa[0] = (1 - f3) * (1 - f1) * (1 - f2);
+            wa[1] = (1 - f3) * (f1) * (1 - f2);
+            wa[2] = (1 - f3) * (1 - f1) * (f2);
+            wa[3] = (1 - f3) * (f1) * (f2);
+            wa[4] = (f3) * (1 - f1) * (1 - f2);
+            wa[5] = (f3) * (f1) * (1 - f2);
+            wa[6] = (f3) * (1 - f1) * (f2);
+            wa[7] = (f3) * (f1) * (f2);
+        }
+    }
+}
+
+/* (1-d)/(1+d) with d = v dt / h in float (abcone3d_make, fdutil.c:1995-2015) */
+float awefd_oneway_cart(float v, float dt, float h)
+{
+    const float dd = v * dt / h;
+    return (1 - dd) / (1 + dd);
+}
+
+/* (1-d)/(1+d) with d = v (dt / h) in float (MAWEFDspher3Dcpu.c:245-270) */
+float awefd_oneway_spher(float v, float dt, float h)
+{
+    const float dd = v * (dt / h);
+    return (1 - dd) / (1 + dd);
+}
+
+static int inj_cmp(const void *pa, const void *pb)
+{
+    const awefd_inj_key *a = (const awefd_inj_key *)pa, *b = (const awefd_inj_key *)pb;
+    if (a->bucket != b->bucket) return a->bucket < b->bucket ? -1 : 1;
+    if (a->e.y != b->e.y) return a->e.y < b->e.y ? -1 : 1;
+    if (a->e.z != b->e.z) return a->e.z < b->e.z ? -1 : 1;
+    if (a->e.x != b->e.x) return a->e.x < b->e.x ? -1 : 1;
+    if (a->e.src != b->e.src) return a->e.src < b->e.src ? -1 : 1;
+    return 0;
+}
+
+/* Sort injection contributions by (bucket, y, z, x, source) so that, at every
+ * node, contributions are added in source order as the reference's serial
+ * loop adds them; ibeg[b]..ibeg[b+1] is bucket b. */
+void awefd_sort_inj(awefd_inj_key *keys, long n, int nbucket, awefd_inj *out, int *ibeg)
+{
+    qsort(keys, (size_t)n, sizeof(*keys), inj_cmp);
+    for (int b = 0; b <= nbucket; b++) ibeg[b] = 0;
+    for (long i = 0; i < n; i++) {
+        out[i] = keys[i].e;
+        ibeg[keys[i].bucket + 1]++;
+    }
+    for (int b = 0; b < nbucket; b++) ibeg[b + 1] += ibeg[b];
+}
