@@ -14,6 +14,26 @@
 
 #define ORC_PI 3.14159265358979323846264338328
 
+#ifdef _OPENMP
+#include <omp.h>
+static double orc_now(void) { return omp_get_wtime(); }
+int orc_threads(void) { return omp_get_max_threads(); }
+#else
+#include <time.h>
+static double orc_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+int orc_threads(void) { return 1; }
+#endif
+
+/* wall time of the last time loop (omp_get_wtime around the loop, as the
+ * reference's timing at MAWEFDcart2Dcpu.c:109-114,384-390) */
+double orc_loop_seconds = 0.0;
+double orc_last_loop_seconds(void) { return orc_loop_seconds; }
+
 /* ------------------------------------------------------------------ */
 /* FD coefficients: MAWEFDcart2Dcpu.c:423-551 (3-D: MAWEFDcart3Dcpu.c:404-531) */
 

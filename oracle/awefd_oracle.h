@@ -87,6 +87,10 @@ int orc_spher3d_f64(const orc_par *p, const float *vel,
                     const float *sou, const float *rec, const float *wav,
                     double *dat, double *ufin, orc_snap_f64 cb, void *ud);
 
+/* Wall time (s) of the last time loop, and the OpenMP thread count. */
+double orc_last_loop_seconds(void);
+int    orc_threads(void);
+
 /* Building blocks exposed for the known-answer tests. */
 int  orc_fdcoef(int nop, int ndim, const float *d, int optfd, int order,
                 float *out /* ndim*nop+1 */);

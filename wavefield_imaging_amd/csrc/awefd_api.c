@@ -98,27 +98,51 @@ void awefd_destroy(awefd_model *m)
     free(m);
 }
 
-/* Tiling of the cartesian step kernel: z tiles of balanced width <= 64,
- * x tiles of 16 rows, y chunks sized so that the launch has >= ~2048 blocks
- * (8 per CU on 256 CUs). */
+/* Tiling of the cartesian step kernel: z tiles of balanced width
+ * <= 64 - 2*nop (a wave also holds nop halo lanes on each side), x tiles of
+ * 16 rows, y chunks sized so that the launch has >= ~2048 blocks (8 per CU on
+ * 256 CUs). */
 static void cart_tiling(awefd_model *m)
 {
     const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2];
+    if (m->target_blocks <= 0) {
+        /* about 16K blocks (12-13 rounds of resident blocks on 256 CUs):
+         * short y chunks keep neighbouring tiles in step, so halo rows are
+         * still in L2 when they are re-read, and the last round is small.
+         * AWEFD_ROUNDS=r asks for r whole rounds instead. */
+        const char *e = getenv("AWEFD_TARGET_BLOCKS");
+        const char *r = getenv("AWEFD_ROUNDS");
+        int cus = 256, dev = m->dev;
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        int per = awefd_cart_step_occupancy(m->nop, m->ndim, m->p.sse);
+        if (per <= 0) per = 4;
+        m->target_blocks = e ? atoi(e) : r ? atoi(r) * per * cus : 16384;
+    }
+    /* z tiles of 64 nodes on 256-byte boundaries */
     m->ntz = (n1 + 63) / 64;
-    m->tz = (n1 + m->ntz - 1) / m->ntz;
+    m->tz = 64;
+    /* the z-high ABC recurrence needs its nop+2 nodes in the last tile */
+    m->zsep = m->p.dabc && m->ndim == 3 && m->ntz > 1 && n1 - 64 * (m->ntz - 1) < m->nop + 2;
     m->ntx = (n2 + AWEFD_CART_TX - 1) / AWEFD_CART_TX;
     if (m->ndim == 3) {
         int cols = m->ntz * m->ntx;
-        int nty = (2048 + cols - 1) / cols;
+        int nty = (m->target_blocks + cols / 2) / cols;
         if (nty > n3) nty = n3;
         if (nty < 1) nty = 1;
+        /* a block addresses (ychunk + 2 nop) planes through 32-bit buffer
+         * offsets: keep that window below 1 GiB */
+        const long plane = (long)m->ld * n2 * 4;
+        while (nty < n3 && ((n3 + nty - 1) / nty + 2L * m->nop) * plane >= (1L << 30)) nty++;
+        m->nty = nty;
         m->ychunk = (n3 + nty - 1) / nty;
-        m->nty = (n3 + m->ychunk - 1) / m->ychunk;
     } else {
         m->ychunk = 1;
         m->nty = 1;
     }
     m->nblk = m->ntz * m->ntx * m->nty;
+    if (getenv("AWEFD_VERBOSE"))
+        fprintf(stderr, "awefd: step kernel tiles %d(z) x %d(x) x %d(y, <= %d planes) = %d blocks, "
+                "target %d\n", m->ntz, m->ntx, m->nty, m->ychunk, m->nblk, m->target_blocks);
 }
 
 int awefd_create(const awefd_params *p, awefd_model **out)
@@ -168,6 +192,9 @@ int awefd_create(const awefd_params *p, awefd_model **out)
         m->opad[a] = m->p.o[a] - (float)nba * m->p.d[a];
         m->ncell *= m->np[a];
     }
+    /* device rows padded to 128 bytes for the cartesian kernels */
+    m->ld = m->p.kind <= AWEFD_CART3D ? (m->np[0] + 31) / 32 * 32 : m->np[0];
+    m->nalloc = (long)m->ld * m->np[1] * m->np[2];
     if (m->ncell > (1L << 33)) {
         free(m);
         return awefd__fail(AWEFD_EINVAL, "grid of %ld nodes too large", m->ncell);
@@ -188,6 +215,15 @@ int awefd_create(const awefd_params *p, awefd_model **out)
         awefd_fdcoef(m->nop, m->ndim, d, p->optfd, 2, m->c2);
         awefd_fdcoef(m->nop, m->ndim, d, p->optfd, 1, m->c1);
         cart_tiling(m);
+        {
+            const long plane = (long)m->ld * m->np[1] * 4;
+            const long win = m->ndim == 3 ? (m->ychunk + 2L * m->nop) * plane : plane;
+            if (win >= (1L << 30)) {
+                awefd_destroy(m);
+                return awefd__fail(AWEFD_EINVAL, "a %dx%d plane is too large for the 1 GiB "
+                                   "buffer window of the step kernel", m->np[0], m->np[1]);
+            }
+        }
         if (m->ndim == 3 && p->dabc && (m->np[0] < 2 * m->nop + 4 || m->np[1] < 2 * m->nop + 4
                                         || m->np[2] < 2 * m->nop + 4)) {
             awefd_destroy(m);
@@ -198,7 +234,7 @@ int awefd_create(const awefd_params *p, awefd_model **out)
             /* variable density handled when a density is supplied */
         }
     }
-    const size_t bytes = sizeof(float) * (size_t)m->ncell;
+    const size_t bytes = sizeof(float) * (size_t)m->nalloc;
     if ((e = hipMalloc((void **)&m->dP, bytes)) != hipSuccess
         || (e = hipMalloc((void **)&m->dC, bytes)) != hipSuccess
         || (e = hipMalloc((void **)&m->dvel, bytes)) != hipSuccess) {
@@ -293,7 +329,8 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
         for (long c = 0; c < (long)n2 * n3; c++)
             for (int iz = 0; iz <= m->nb && iz < n1; iz++) v[c * n1 + iz] = 0.f;
     if (rc == 0) {
-        hipError_t e = hipMemcpy(m->dvel, v, sizeof(float) * m->ncell, hipMemcpyHostToDevice);
+        hipError_t e = hipMemcpy2D(m->dvel, sizeof(float) * m->ld, v, sizeof(float) * n1,
+                                   sizeof(float) * n1, (size_t)n2 * n3, hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "vel upload: %s", hipGetErrorString(e));
     }
     free(v);
@@ -442,9 +479,9 @@ int awefd_reset(awefd_model *m)
 {
     if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
     HIPCHK(hipSetDevice(m->dev));
-    int e = awefd_launch_fill(m->dP, m->ncell, 0.f, m->stream);
-    if (!e) e = awefd_launch_fill(m->dC, m->ncell, 0.f, m->stream);
-    if (!e && m->dF) e = awefd_launch_fill(m->dF, m->ncell, 0.f, m->stream);
+    int e = awefd_launch_fill(m->dP, m->nalloc, 0.f, m->stream);
+    if (!e) e = awefd_launch_fill(m->dC, m->nalloc, 0.f, m->stream);
+    if (!e && m->dF) e = awefd_launch_fill(m->dF, m->nalloc, 0.f, m->stream);
     if (e) return awefd__fail(AWEFD_EHIP, "reset: %s", hipGetErrorString((hipError_t)e));
     m->steps_done = 0;
     return AWEFD_OK;
@@ -492,6 +529,13 @@ static int prof_collect(awefd_model *m, int npairs)
     return AWEFD_OK;
 }
 
+/* Device field (row stride ld) to a dense host array (row stride np[0]). */
+static hipError_t copy_field_out(const awefd_model *m, float *host, const float *dev)
+{
+    return hipMemcpy2D(host, sizeof(float) * m->np[0], dev, sizeof(float) * m->ld,
+                       sizeof(float) * m->np[0], (size_t)m->np[1] * m->np[2], hipMemcpyDeviceToHost);
+}
+
 /* Apply the pending sponge factor to a host copy of the stored field (the
  * reference's u0 view), in the reference's z, x, y order. */
 static void cart_apply_pending(const awefd_model *m, float *u)
@@ -518,7 +562,7 @@ int awefd_get_field(awefd_model *m, float *host)
     HIPCHK(hipSetDevice(m->dev));
     if (!is_cart(m)) return awefd__spher_get_field(m, host);
     HIPCHK(hipStreamSynchronize(m->stream));
-    HIPCHK(hipMemcpy(host, m->dP, sizeof(float) * m->ncell, hipMemcpyDeviceToHost));
+    HIPCHK(copy_field_out(m, host, m->dP));
     cart_apply_pending(m, host);
     return AWEFD_OK;
 }
@@ -531,6 +575,8 @@ static void cart_args(const awefd_model *m, awefd_cart_args *a)
     a->n1 = m->np[0];
     a->n2 = m->np[1];
     a->n3 = m->np[2];
+    a->ld = m->ld;
+    a->zsep = m->zsep;
     const int nop = m->nop;
     a->c0 = m->c2[0];
     for (int k = 1; k <= nop; k++) {
@@ -624,7 +670,7 @@ static int cart_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, vo
         if (cb && jsnap > 0 && it % jsnap == 0) {
             if (!host && !(host = malloc(sizeof(float) * m->ncell))) return awefd__fail(AWEFD_ENOMEM, "snapshot buffer");
             HIPCHK(hipStreamSynchronize(m->stream));
-            HIPCHK(hipMemcpy(host, P, sizeof(float) * m->ncell, hipMemcpyDeviceToHost));
+            HIPCHK(copy_field_out(m, host, P));
             cart_apply_pending(m, host);
             cb(user, it, host, m->np);
         }

@@ -13,7 +13,13 @@ extern "C" {
 #endif
 
 #define AWEFD_MAX_NOP 6
-#define AWEFD_CART_TX 16          /* x rows per stencil block (4 waves x 4 rows) */
+#ifndef AWEFD_RX
+#define AWEFD_RX 4                /* x rows per wave of the cartesian step kernel */
+#endif
+#ifndef AWEFD_NWAVE
+#define AWEFD_NWAVE 4             /* waves per block of the cartesian step kernel */
+#endif
+#define AWEFD_CART_TX (AWEFD_NWAVE * AWEFD_RX) /* x rows per stencil block     */
 
 /* One source-injection contribution: node (z, x, y) += amp(src) * w * vel. */
 typedef struct {
@@ -29,11 +35,13 @@ typedef struct {
     const float *vel;        /* (v dt)^2, padded                              */
     const float *rho;        /* density (variable-density runs) or NULL       */
     int n1, n2, n3;          /* padded z, x, y                                */
+    int ld;                  /* row stride in floats (n1 rounded up to 32)    */
+    int zsep;                /* z-high ABC done by a separate pass            */
     float c0;
     float cz[AWEFD_MAX_NOP], cx[AWEFD_MAX_NOP], cy[AWEFD_MAX_NOP];
     float bz[AWEFD_MAX_NOP], bx[AWEFD_MAX_NOP], by[AWEFD_MAX_NOP];
     /* tiling */
-    int tz, ntz, ntx, ychunk, nty, nblk;
+    int tz, ntz, ntx, ychunk, nty, nblk; /* ychunk: longest y chunk      */
     /* sponge: per-axis factors, 1 outside the strips */
     int dabc, nb;
     const float *sp1, *sp2, *sp3;
@@ -59,6 +67,8 @@ int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndim, int sse,
                            void *stream);
 int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void *stream);
 int awefd_launch_fill(float *p, long n, float v, void *stream);
+/* resident blocks per CU of the cartesian step kernel (0 if unknown) */
+int awefd_cart_step_occupancy(int nop, int ndim, int sse);
 
 #ifdef __cplusplus
 }

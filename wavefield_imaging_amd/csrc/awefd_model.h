@@ -14,6 +14,8 @@ struct awefd_model {
     int nop;            /* stencil half-width (cartesian) / NOP=4 (spherical) */
     int nb;             /* effective padding                                 */
     int np[3];          /* padded dims, fastest first (model axis order)     */
+    int ld;             /* device row stride in floats (>= np[0])            */
+    long nalloc;        /* device floats per field: ld * np[1] * np[2]       */
     float opad[3];      /* padded origins                                    */
     long ncell;
     int model_set, src_set, rec_set, wav_set;
@@ -45,6 +47,8 @@ struct awefd_model {
 
     /* cartesian tiling */
     int tz, ntz, ntx, ychunk, nty, nblk;
+    int target_blocks;            /* stencil blocks the y chunking aims at   */
+    int zsep;                     /* z-high one-way ABC as a separate pass   */
 
     /* spherical state (spher_host.c) */
     double *d_damp;

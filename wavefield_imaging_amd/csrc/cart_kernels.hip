@@ -19,12 +19,22 @@
  * Every multiplication happens in the reference order, so the result is bit
  * for bit the reference's.
  *
- * Layout: z fastest (RSF axis 1), then x, then y; row length n1.
- * Blocks: 256 threads = 4 waves; a wave covers up to 64 z nodes of 4 x rows;
- * a block covers a (tz <= 64) x 16 tile and streams it along y (2.5-D
- * blocking) with a per-thread register queue of 2*NOP+1 planes.  The current
- * plane of the tile plus NOP-wide halos is staged in LDS (double buffered,
- * one barrier per plane).  Each field value is read from HBM once per step.
+ * Layout: z fastest (RSF axis 1), then x, then y.  Rows are padded to ld
+ * nodes, a multiple of 32 (128 bytes), so that a z tile of 64 nodes is two
+ * whole cache lines: no line is shared between tiles.
+ *
+ * Step kernel: NWAVE waves per block.  Lane l of a wave holds z = z0 + l of
+ * RX x rows; a block covers 64 z x NWAVE*RX x nodes and streams them along y
+ * (2.5-D blocking).  z neighbours are moved across lanes by DPP wave shifts,
+ * the 2*NOP values beyond the tile edges arriving with one masked load per
+ * row; x neighbours come from the wave's own rows or from loads of the
+ * adjacent rows that hit L1/L2; y neighbours sit in a register ring.  Each
+ * field value is read from HBM once per step.
+ *
+ * Addressing: AMD buffer instructions.  The per-lane part of an offset (z)
+ * is loop-invariant; the row/plane part is a scalar offset.  Lanes or rows
+ * that must not touch memory get offsets beyond the descriptor's range,
+ * which the hardware turns into zero loads and dropped stores.
  *
  * Compiled with -ffp-contract=off: no FMA contraction, so every add and
  * multiply rounds as in the reference's SSE (or scalar) build.
@@ -37,8 +47,16 @@
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int NWAVE = 4;
-constexpr int RX = 4;
+constexpr int NWAVE = AWEFD_NWAVE;    /* waves per block */
+constexpr int NTHR = NWAVE * WAVE;
+constexpr int RX = AWEFD_RX;          /* x rows per wave */
+/* cache policy of the streamed arrays (previous field, (v dt)^2, stores):
+ * non-temporal, so that L2 keeps the current field's rows for their reuse
+ * as halos */
+#ifndef AWEFD_NT
+#define AWEFD_NT 2
+#endif
+constexpr int NT = AWEFD_NT;
 constexpr int TX = NWAVE * RX;
 static_assert(TX == AWEFD_CART_TX, "tile width");
 
@@ -49,6 +67,37 @@ __device__ __forceinline__ int xcd_remap(int b, int n)
 {
     const int per = n >> 3, rem = n & 7, xcd = b & 7, slot = b >> 3;
     return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr unsigned OOB = 0x80000000u;
+
+__device__ __forceinline__ rsrc_t mk_rsrc(const float *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+/* voffset: per-lane bytes; soffset: wave-uniform bytes (both range-checked) */
+template <int POL = 0>
+__device__ __forceinline__ float bld(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, POL));
+}
+template <int POL = 0>
+__device__ __forceinline__ void bst(rsrc_t r, unsigned voff, unsigned soff, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, POL);
+}
+/* DPP wave shifts: lane i receives lane i-1 (shr) or lane i+1 (shl); the
+ * lane without a source (0 for shr, 63 for shl) keeps `old`. */
+__device__ __forceinline__ float shr1(float old, float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float shl1(float old, float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
 }
 
 template <int NDIM>
@@ -74,7 +123,7 @@ template <int NDIM>
 __device__ void cart_extract(const awefd_cart_args &a, int r)
 {
     if (r >= a.nr) return;
-    const long sx = a.n1, sy = (long)a.n1 * a.n2;
+    const long sx = a.ld, sy = (long)a.ld * a.n2;
     const int jz = a.rj[3 * r], jx = a.rj[3 * r + 1], jy = a.rj[3 * r + 2];
     constexpr int NC = NDIM == 3 ? 8 : 4;
     float s = 0.f;
@@ -89,12 +138,57 @@ __device__ void cart_extract(const awefd_cart_args &a, int r)
     a.trow[r] = s;
 }
 
-template <int NOP, int NDIM, bool SSE>
-__global__ __launch_bounds__(256) void k_cart_step(const awefd_cart_args a)
+/* x neighbour at offset `off` of row j of a wave: its own rows, else the
+ * halo rows (h < NOP: x0-NOP+h, h >= NOP: x0+RX+h-NOP).  j and off are
+ * compile-time constants after unrolling. */
+template <int NOP>
+__device__ __forceinline__ float xnb(const float (&cen)[RX], const float (&hal)[2 * NOP], int j, int off)
 {
+    const int r = j + off;
+    if (r >= 0 && r < RX) return cen[r];
+    return r < 0 ? hal[NOP + r] : hal[NOP + r - RX];
+}
+
+/* byte offset of row x at plane y relative to the block window (wave-uniform;
+ * rows/planes below the window wrap to offsets beyond its range) */
+__device__ __forceinline__ unsigned soff(int x, int y, int ybase, unsigned rowb, unsigned psz)
+{
+    return (unsigned)x * rowb + (unsigned)(y - ybase) * psz;
+}
+
+template <int NOP>
+__device__ __forceinline__ void load_pvh(float (&p)[RX], float (&v)[RX], float (&zh)[RX],
+                                         float (&hal)[2 * NOP], rsrc_t rP, rsrc_t rV, rsrc_t rC,
+                                         unsigned vz, unsigned vzh, int x0, int y, int ybase,
+                                         unsigned rowb, unsigned psz)
+{
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const unsigned s = soff(x0 + j, y, ybase, rowb, psz);
+        p[j] = bld<NT>(rP, vz, s);
+        v[j] = bld<NT>(rV, vz, s);
+        zh[j] = bld(rC, vzh, s);
+    }
+#pragma unroll
+    for (int h = 0; h < 2 * NOP; h++) {
+        const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
+        hal[h] = bld(rC, vz, soff(hx, y, ybase, rowb, psz));
+    }
+}
+
+/* The step kernel.  Planes of the current field live in a ring of
+ * QN = 2*NOP+2 register slots; the previous field, (v dt)^2 and the halo
+ * values of the next plane in the second of two register buffers; the y loop
+ * is unrolled QN times so that every slot index is a compile-time constant.
+ * The loads of plane y+1 are in flight while plane y is computed.  No LDS and
+ * no barriers: waves advance independently. */
+template <int NOP, int NDIM, bool SSE>
+__global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
+{
+    constexpr int QN = 2 * NOP + 2;
     const int tid = threadIdx.x;
     if ((int)blockIdx.x >= a.nblk) {
-        cart_extract<NDIM>(a, ((int)blockIdx.x - a.nblk) * 256 + tid);
+        cart_extract<NDIM>(a, ((int)blockIdx.x - a.nblk) * NTHR + tid);
         return;
     }
     const int lb = xcd_remap(blockIdx.x, a.nblk);
@@ -102,39 +196,71 @@ __global__ __launch_bounds__(256) void k_cart_step(const awefd_cart_args a)
     const int rr = lb / a.ntz;
     const int txi = rr % a.ntx;
     const int tyi = rr / a.ntx;
-    const int lane = tid & (WAVE - 1), wv = tid / WAVE;
-    const int z0 = tzi * a.tz;
-    const int tzw = min(a.tz, a.n1 - z0);
-    const int x0 = txi * TX;
-    const int y0 = NDIM == 3 ? tyi * a.ychunk : 0;
-    const int y1 = NDIM == 3 ? min(y0 + a.ychunk, a.n3) : 1;
+    const int lane = tid & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+    const int z0 = tzi * WAVE;
     const int iz = z0 + lane;
-    const bool zin = lane < tzw;
-    const long sx = a.n1, sy = (long)a.n1 * a.n2;
+    const bool zin = iz < n1;
+    const int x0 = txi * TX + wv * RX;                   /* wave-uniform */
+    /* y chunks of balanced length (nty chunks over n3 planes) */
+    const int y0 = NDIM == 3 ? (int)((long)tyi * n3 / a.nty) : 0;
+    const int y1 = NDIM == 3 ? (int)((long)(tyi + 1) * n3 / a.nty) : 1;
+    const long sy = (long)a.ld * n2;
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    const unsigned psz = (unsigned)(sy * 4);
 
-    constexpr int LW = WAVE + 2 * NOP;
-    constexpr int LR = TX + 2 * NOP;
-    __shared__ float lds[2][LR * LW];
+    /* this block's window of planes [ybase, ytop), < 1 GiB (host-checked) */
+    const int ybase = NDIM == 3 ? max(y0 - NOP, 0) : 0;
+    const int ytop = NDIM == 3 ? min(y1 + NOP, n3) : 1;
+    const unsigned nbytes = (unsigned)(ytop - ybase) * psz;
+    const rsrc_t rC = mk_rsrc(a.C + ybase * sy, nbytes);
+    const rsrc_t rP = mk_rsrc(a.P + ybase * sy, nbytes);
+    const rsrc_t rV = mk_rsrc(a.vel + ybase * sy, nbytes);
 
-    int ixr[RX];
-    bool xin[RX];
+    /* per-lane offsets: the node itself, and the z value beyond the tile edge
+     * this lane fetches (lane k < NOP: z0-1-k; lane 63-k: z0+64+k) */
+    const unsigned vz = zin ? (unsigned)iz * 4u : OOB;
+    int zz = -1;
+    if (lane < NOP) zz = z0 - 1 - lane;
+    else if (lane >= WAVE - NOP) zz = z0 + WAVE + (WAVE - 1 - lane);
+    const unsigned vzh = (zz >= 0 && zz < n1) ? (unsigned)zz * 4u : OOB;
+
+    /* per-lane constants */
+    const bool zint = iz >= NOP && iz < n1 - NOP;
+    const bool zsp = a.dabc && zin && (iz < a.nb || iz >= n1 - a.nb);
+    const float spz = zsp ? a.sp1[iz] : 1.f;
+    const bool zsp_w = a.dabc && (z0 < a.nb || z0 + WAVE > n1 - a.nb);   /* wave-uniform */
+    const bool zdef = a.zsep && iz >= n1 - NOP - 2;                      /* z-high shell */
+    /* per-row (wave-uniform) constants, as bit masks over the RX rows */
+    float spx[RX];
+    unsigned xint_m = 0, xdef_m = 0, xin_m = 0;
+    bool xsp_w = false;
 #pragma unroll
     for (int j = 0; j < RX; j++) {
-        ixr[j] = x0 + wv * RX + j;
-        xin[j] = ixr[j] < a.n2;
+        const int ix = x0 + j;
+        const bool xsp = a.dabc && ix < n2 && (ix < a.nb || ix >= n2 - a.nb);
+        xsp_w = xsp_w || xsp;
+        spx[j] = xsp ? a.sp2[ix] : 1.f;
+        xin_m |= ix < n2 ? 1u << j : 0u;
+        xint_m |= (ix >= NOP && ix < n2 - NOP) ? 1u << j : 0u;
+        xdef_m |= (NDIM == 3 && a.abc && (ix <= NOP + 1 || ix >= n2 - NOP - 2)) ? 1u << j : 0u;
     }
+    const bool sp_xz = zsp_w || xsp_w;
 
-    float q[RX][2 * NOP + 1];
+    /* ring: current-field plane p in slot (p - y0 + NOP) % QN */
+    float q[RX][QN];
+    float bp_[2][RX], bv_[2][RX], bz_[2][RX], bh_[2][2 * NOP];
     if (NDIM == 3) {
 #pragma unroll
-        for (int k = 0; k < 2 * NOP; k++) {
-            const int yy = y0 - NOP + k;
+        for (int k = 0; k <= 2 * NOP; k++)
 #pragma unroll
-            for (int j = 0; j < RX; j++)
-                q[j][k + 1] = (zin && xin[j] && yy >= 0 && yy < a.n3)
-                                  ? a.C[yy * sy + ixr[j] * sx + iz] : 0.f;
-        }
+            for (int j = 0; j < RX; j++) q[j][k] = bld(rC, vz, soff(x0 + j, y0 - NOP + k, ybase, rowb, psz));
+    } else {
+#pragma unroll
+        for (int j = 0; j < RX; j++) q[j][0] = bld(rC, vz, soff(x0 + j, 0, 0, rowb, psz));
     }
+    load_pvh<NOP>(bp_[0], bv_[0], bz_[0], bh_[0], rP, rV, rC, vz, vzh, x0, y0, ybase, rowb, psz);
 
     /* injection entries of this tile column, restricted to planes >= y0 */
     const int col = tzi + a.ntz * txi;
@@ -143,154 +269,173 @@ __global__ __launch_bounds__(256) void k_cart_step(const awefd_cart_args a)
     while (ib < ie && a.inj[ib].y < y0) ib++;
 
     const bool zlow = NDIM == 3 && a.abc && tzi == 0 && !a.fsrf;
-    const bool zhigh = NDIM == 3 && a.abc && tzi == a.ntz - 1;
-    int buf = 0;
-    for (int y = y0; y < y1; y++, buf ^= 1) {
-        float *L = lds[buf];
-        float cen[RX];
-        if (NDIM == 3) {
-            const int yn = y + NOP;
+    const bool zhigh = NDIM == 3 && a.abc && !a.zsep && tzi == a.ntz - 1;
+
+    for (int yg = y0; yg < y1; yg += QN) {
+#pragma unroll
+        for (int R = 0; R < QN; R++) {
+            const int y = yg + R;
+            if (y >= y1) break;
+            const int S0 = NDIM == 3 ? R : 0;     /* slot of plane y-NOP */
+            const int B = R & 1;                  /* buffer of plane y   */
+            /* loads of plane y+1: head y+1+NOP into the slot of plane
+             * y-NOP-1, previous field / vel / halos into the other buffer */
+            if (y + 1 < y1) {
+                if (NDIM == 3) {
+#pragma unroll
+                    for (int j = 0; j < RX; j++)
+                        q[j][(S0 + 2 * NOP + 1) % QN] = bld(rC, vz, soff(x0 + j, y + 1 + NOP, ybase, rowb, psz));
+                }
+                load_pvh<NOP>(bp_[B ^ 1], bv_[B ^ 1], bz_[B ^ 1], bh_[B ^ 1], rP, rV, rC, vz, vzh, x0,
+                              y + 1, ybase, rowb, psz);
+            }
+            float cen[RX];
+#pragma unroll
+            for (int j = 0; j < RX; j++) cen[j] = NDIM == 3 ? q[j][(S0 + NOP) % QN] : q[j][0];
+
+            const bool yint = NDIM == 2 || (y >= NOP && y < n3 - NOP);
+            const bool ysp = NDIM == 3 && a.dabc && (y < a.nb || y >= n3 - a.nb);
+            const float spy = ysp ? a.sp3[y] : 1.f;
+            const bool ysh = NDIM == 3 && a.abc && (y <= NOP + 1 || y >= n3 - NOP - 2);
+
+            float nv[RX], zp1[RX], zm1[RX];
 #pragma unroll
             for (int j = 0; j < RX; j++) {
+                /* z neighbours: adjacent lanes, the edge lanes taking the
+                 * fetched values beyond the tile (moved to lane 0 / 63) */
+                float zm[NOP], zp[NOP];
+                float hlo = bz_[B][j], hhi = bz_[B][j];
+                zm[0] = shr1(hlo, cen[j]);
+                zp[0] = shl1(hhi, cen[j]);
 #pragma unroll
-                for (int k = 0; k < 2 * NOP; k++) q[j][k] = q[j][k + 1];
-                q[j][2 * NOP] = (zin && xin[j] && yn < a.n3) ? a.C[yn * sy + ixr[j] * sx + iz] : 0.f;
-                cen[j] = q[j][NOP];
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < RX; j++) cen[j] = (zin && xin[j]) ? a.C[ixr[j] * sx + iz] : 0.f;
-        }
-        const long yb = (long)y * sy;
-        float pv[RX], vv[RX];
-#pragma unroll
-        for (int j = 0; j < RX; j++) {
-            pv[j] = 0.f;
-            vv[j] = 0.f;
-            if (zin && xin[j]) {
-                pv[j] = a.P[yb + ixr[j] * sx + iz];
-                vv[j] = a.vel[yb + ixr[j] * sx + iz];
-            }
-        }
-        /* stage the current plane (tile + NOP halos) in LDS */
-        if (zin) {
-#pragma unroll
-            for (int j = 0; j < RX; j++) L[(wv * RX + j + NOP) * LW + lane + NOP] = cen[j];
-        }
-        for (int h = wv; h < 2 * NOP; h += NWAVE) {
-            const int xr = h < NOP ? x0 - NOP + h : x0 + TX + (h - NOP);
-            const int lrow = h < NOP ? h : TX + NOP + (h - NOP);
-            if (zin) {
-                float v = 0.f;
-                if (xr >= 0 && xr < a.n2) v = a.C[yb + xr * sx + iz];
-                L[lrow * LW + lane + NOP] = v;
-            }
-        }
-        for (int e = tid; e < 2 * NOP * TX; e += 256) {
-            const int row = e / (2 * NOP), s = e % (2 * NOP);
-            const int xr = x0 + row;
-            const int zz = s < NOP ? z0 - NOP + s : z0 + tzw + (s - NOP);
-            const int lcol = s < NOP ? s : NOP + tzw + (s - NOP);
-            float v = 0.f;
-            if (xr < a.n2 && zz >= 0 && zz < a.n1) v = a.C[yb + xr * sx + zz];
-            L[(row + NOP) * LW + lcol] = v;
-        }
-        __syncthreads();
-
-        const bool yint = NDIM == 2 || (y >= NOP && y < a.n3 - NOP);
-        const bool zint = iz >= NOP && iz < a.n1 - NOP;
-        float nv[RX];
-#pragma unroll
-        for (int j = 0; j < RX; j++) {
-            const int ix = ixr[j];
-            float p = pv[j];
-            if (a.dabc && zin && xin[j] && in_sponge<NDIM>(iz, ix, y, a)) p = sponge<NDIM>(p, iz, ix, y, a);
-            const bool interior = zint && yint && ix >= NOP && ix < a.n2 - NOP;
-            const float *Lr = L + (wv * RX + j + NOP) * LW + lane + NOP;
-            float v = p;
-            if (zin && xin[j] && interior) {
+                for (int k = 1; k < NOP; k++) {
+                    hlo = shl1(hlo, hlo);
+                    hhi = shr1(hhi, hhi);
+                    zm[k] = shr1(hlo, zm[k - 1]);
+                    zp[k] = shl1(hhi, zp[k - 1]);
+                }
+                zp1[j] = zp[0];
+                zm1[j] = zm[0];
+                /* the pending sponge factor of the previous field */
+                float p = bp_[B][j];
+                if (sp_xz || ysp) {   /* wave-uniform: no divergence */
+                    p = p * spz;
+                    p = p * spx[j];
+                    if (NDIM == 3) p = p * spy;
+                }
+                /* every lane evaluates the stencil, then selects */
                 const float c = cen[j];
-                float lap = c * a.c0;
+                float lap = c * a.c0, v;
                 if (SSE) {
 #pragma unroll
                     for (int k = 1; k <= NOP; k++) {
-                        lap = lap + a.cz[k - 1] * (Lr[k] + Lr[-k]);
-                        lap = lap + a.cx[k - 1] * (Lr[k * LW] + Lr[-k * LW]);
-                        if (NDIM == 3) lap = lap + a.cy[k - 1] * (q[j][NOP + k] + q[j][NOP - k]);
+                        const float xpk = xnb<NOP>(cen, bh_[B], j, k), xmk = xnb<NOP>(cen, bh_[B], j, -k);
+                        lap = lap + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]);
+                        lap = lap + a.cx[k - 1] * (xpk + xmk);
+                        if (NDIM == 3)
+                            lap = lap + a.cy[k - 1] * (q[j][(S0 + NOP + k) % QN] + q[j][(S0 + NOP - k) % QN]);
                     }
                     const float t = 2.f * c - p;
-                    v = t + vv[j] * lap;
+                    v = t + bv_[B][j] * lap;
                 } else {
 #pragma unroll
                     for (int k = 1; k <= NOP; k++) {
-                        float t = (Lr[-k] + Lr[k]) * a.cz[k - 1] + (Lr[-k * LW] + Lr[k * LW]) * a.cx[k - 1];
-                        if (NDIM == 3) t = t + (q[j][NOP - k] + q[j][NOP + k]) * a.cy[k - 1];
+                        const float xpk = xnb<NOP>(cen, bh_[B], j, k), xmk = xnb<NOP>(cen, bh_[B], j, -k);
+                        float t = (zm[k - 1] + zp[k - 1]) * a.cz[k - 1] + (xmk + xpk) * a.cx[k - 1];
+                        if (NDIM == 3)
+                            t = t + (q[j][(S0 + NOP - k) % QN] + q[j][(S0 + NOP + k) % QN]) * a.cy[k - 1];
                         lap = lap + t;
                     }
-                    const float vl = vv[j] * lap;
+                    const float vl = bv_[B][j] * lap;
                     v = (float)(2.0 * (double)c - (double)p + (double)vl);
                 }
+                nv[j] = (zint && ((xint_m >> j) & 1u) && yint) ? v : p;
             }
-            nv[j] = v;
-        }
 
-        /* source injection into the new field, in source order */
-        int e = ib;
-        for (; e < ie && a.inj[e].y == y; e++) {
-            const awefd_inj en = a.inj[e];
-            const float amp = a.wav[(long)a.it * a.wstride + (a.wexpl ? 0 : en.src)];
+            /* source injection into the new field, in source order */
+            int e = ib;
+            for (; e < ie && a.inj[e].y == y; e++) {
+                const awefd_inj en = a.inj[e];
+                const float amp = a.wav[(long)a.it * a.wstride + (a.wexpl ? 0 : en.src)];
 #pragma unroll
-            for (int j = 0; j < RX; j++)
-                if (zin && en.z == iz && en.x == ixr[j]) nv[j] = nv[j] + amp * en.w * vv[j];
-        }
-        ib = e;
+                for (int j = 0; j < RX; j++) {
+                    const float inj = nv[j] + amp * en.w * bv_[B][j];
+                    nv[j] = (zin && en.z == iz && en.x == x0 + j) ? inj : nv[j];
+                }
+            }
+            ib = e;
 
-        /* one-way ABC on the z faces: serial outward recurrence along z,
-         * carried across lanes (abcone3d_apply, fdutil.c:2097-2120) */
-        if (zlow || zhigh) {
+            /* one-way ABC on the z faces: serial outward recurrence along z,
+             * carried across lanes (abcone3d_apply, fdutil.c:2097-2120); um
+             * is the current field (this lane's centre and z+-1 neighbours) */
+            if (zlow || zhigh) {
+#pragma unroll
+                for (int j = 0; j < RX; j++) {
+                    if (!((xin_m >> j) & 1u)) continue;
+                    if (zlow) {
+                        const float b = a.bzl[y * n2 + x0 + j];
+#pragma unroll
+                        for (int k = 0; k < NOP; k++) {
+                            const int zt = NOP - k;
+                            const float up = shl1(nv[j], nv[j]);
+                            const float w = zp1[j] + (cen[j] - up) * b;
+                            nv[j] = iz == zt ? w : nv[j];
+                        }
+                    }
+                    if (zhigh) {
+                        const float b = a.bzh[y * n2 + x0 + j];
+#pragma unroll
+                        for (int k = 0; k < NOP; k++) {
+                            const int zt = n1 - NOP + k - 1;
+                            const float dn = shr1(nv[j], nv[j]);
+                            const float w = zm1[j] + (cen[j] - dn) * b;
+                            nv[j] = iz == zt ? w : nv[j];
+                        }
+                    }
+                }
+            }
+
+            /* sponge of the new field and store; nodes another face pass
+             * still has to update are stored unsponged */
 #pragma unroll
             for (int j = 0; j < RX; j++) {
-                if (!xin[j]) continue;
-                const int ix = ixr[j];
-                const float *Lc = L + (wv * RX + j + NOP) * LW + NOP;
-                if (zlow) {
-                    const float b = a.bzl[y * a.n2 + ix];
-                    for (int k = 0; k < NOP; k++) {
-                        const int zt = NOP - k;
-                        const float up = __shfl_down(nv[j], 1);
-                        if (lane == zt) nv[j] = Lc[zt + 1] + (Lc[zt] - up) * b;
-                    }
+                if (!((xin_m >> j) & 1u)) continue;
+                float v = nv[j];
+                if ((sp_xz || ysp) && !(((xdef_m >> j) & 1u) || ysh)) {   /* wave-uniform */
+                    float w = v * spz;
+                    w = w * spx[j];
+                    if (NDIM == 3) w = w * spy;
+                    v = zdef ? v : w;
                 }
-                if (zhigh) {
-                    const float b = a.bzh[y * a.n2 + ix];
-                    for (int k = 0; k < NOP; k++) {
-                        const int zt = a.n1 - NOP + k - 1 - z0;
-                        const float dn = __shfl_up(nv[j], 1);
-                        if (lane == zt) nv[j] = Lc[zt - 1] + (Lc[zt] - dn) * b;
-                    }
-                }
+                bst<NT>(rP, vz, soff(x0 + j, y, ybase, rowb, psz), v);
             }
-        }
-
-        /* sponge of the new field and store; ABC x/y shell nodes are left
-         * unsponged for the face kernels */
-#pragma unroll
-        for (int j = 0; j < RX; j++) {
-            if (!(zin && xin[j])) continue;
-            const int ix = ixr[j];
-            float v = nv[j];
-            if (a.dabc && in_sponge<NDIM>(iz, ix, y, a)) {
-                bool defer = false;
-                if (NDIM == 3 && a.abc) {
-                    const bool xsh = ix <= NOP + 1 || ix >= a.n2 - NOP - 2;
-                    const bool ysh = y <= NOP + 1 || y >= a.n3 - NOP - 2;
-                    defer = xsh || ysh;
-                }
-                if (!defer) v = sponge<NDIM>(v, iz, ix, y, a);
-            }
-            a.P[yb + ix * sx + iz] = v;
         }
     }
+}
+
+/* z-high one-way ABC as a separate pass, for grids whose last z tile is too
+ * narrow to hold the recurrence (abcone3d_apply, fdutil.c:2110-2118).  One
+ * thread per (x, y) column. */
+template <int NOP>
+__global__ __launch_bounds__(256) void k_cart_abc_zhigh(const awefd_cart_args a)
+{
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long)a.n2 * a.n3) return;
+    const int ix = (int)(t % a.n2), iy = (int)(t / a.n2);
+    const long b = (long)iy * a.ld * a.n2 + (long)ix * a.ld;
+    const float bh = a.bzh[(long)iy * a.n2 + ix];
+    float *P = a.P;
+    const float *C = a.C;
+    for (int k = 0; k < NOP; k++) {
+        const int iz = a.n1 - NOP + k - 1;
+        P[b + iz] = C[b + iz - 1] + (C[b + iz] - P[b + iz - 1]) * bh;
+    }
+    if (!a.dabc) return;
+    const bool xsh = ix <= NOP + 1 || ix >= a.n2 - NOP - 2;
+    const bool ysh = iy <= NOP + 1 || iy >= a.n3 - NOP - 2;
+    if (xsh || ysh) return;
+    for (int iz = a.n1 - NOP - 2; iz < a.n1; iz++)
+        if (in_sponge<3>(iz, ix, iy, a)) P[b + iz] = sponge<3>(P[b + iz], iz, ix, iy, a);
 }
 
 /* One-way ABC on the x faces, then the y faces (abcone3d_apply,
@@ -302,7 +447,7 @@ __global__ __launch_bounds__(256) void k_cart_abc_x(const awefd_cart_args a)
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)a.n1 * a.n3) return;
     const int iz = (int)(t % a.n1), iy = (int)(t / a.n1);
-    const long sx = a.n1, sy = (long)a.n1 * a.n2;
+    const long sx = a.ld, sy = (long)a.ld * a.n2;
     const long b = iy * sy + iz;
     const float bl = a.bxl[(long)iy * a.n1 + iz], bh = a.bxh[(long)iy * a.n1 + iz];
     float *P = a.P;
@@ -328,7 +473,7 @@ __global__ __launch_bounds__(256) void k_cart_abc_y(const awefd_cart_args a)
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)a.n1 * a.n2) return;
     const int iz = (int)(t % a.n1), ix = (int)(t / a.n1);
-    const long sx = a.n1, sy = (long)a.n1 * a.n2;
+    const long sx = a.ld, sy = (long)a.ld * a.n2;
     const long b = ix * sx + iz;
     const float bl = a.byl[(long)ix * a.n1 + iz], bh = a.byh[(long)ix * a.n1 + iz];
     float *P = a.P;
@@ -355,9 +500,25 @@ __global__ void k_fill(float *p, long n, float v)
 template <int NOP, int NDIM, bool SSE>
 int launch_step(const awefd_cart_args *a, hipStream_t s)
 {
-    const int nextract = (a->nr + 255) / 256;
-    hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE>), dim3(a->nblk + nextract), dim3(256), 0, s, *a);
+    const int nextract = (a->nr + NTHR - 1) / NTHR;
+    hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE>), dim3(a->nblk + nextract), dim3(NTHR), 0, s, *a);
     return (int)hipGetLastError();
+}
+
+template <int NOP, int NDIM, bool SSE>
+int occupancy_step()
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cart_step<NOP, NDIM, SSE>, NTHR, 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+template <int NOP>
+int occupancy_nop(int ndim, int sse)
+{
+    if (ndim == 2) return sse ? occupancy_step<NOP, 2, true>() : occupancy_step<NOP, 2, false>();
+    return sse ? occupancy_step<NOP, 3, true>() : occupancy_step<NOP, 3, false>();
 }
 
 template <int NOP>
@@ -370,6 +531,10 @@ int launch_step_nop(const awefd_cart_args *a, int ndim, int sse, hipStream_t s)
 template <int NOP>
 int launch_abc(const awefd_cart_args *a, hipStream_t s)
 {
+    if (a->zsep) {
+        const long nzl = (long)a->n2 * a->n3;
+        hipLaunchKernelGGL((k_cart_abc_zhigh<NOP>), dim3((unsigned)((nzl + 255) / 256)), dim3(256), 0, s, *a);
+    }
     const long nxl = (long)a->n1 * a->n3, nyl = (long)a->n1 * a->n2;
     hipLaunchKernelGGL((k_cart_abc_x<NOP>), dim3((unsigned)((nxl + 255) / 256)), dim3(256), 0, s, *a);
     hipLaunchKernelGGL((k_cart_abc_y<NOP>), dim3((unsigned)((nyl + 255) / 256)), dim3(256), 0, s, *a);
@@ -391,6 +556,19 @@ extern "C" int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndi
     case 5: return launch_step_nop<5>(a, ndim, sse, s);
     case 6: return launch_step_nop<6>(a, ndim, sse, s);
     default: return (int)hipErrorInvalidValue;
+    }
+}
+
+extern "C" int awefd_cart_step_occupancy(int nop, int ndim, int sse)
+{
+    switch (nop) {
+    case 1: return occupancy_nop<1>(ndim, sse);
+    case 2: return occupancy_nop<2>(ndim, sse);
+    case 3: return occupancy_nop<3>(ndim, sse);
+    case 4: return occupancy_nop<4>(ndim, sse);
+    case 5: return occupancy_nop<5>(ndim, sse);
+    case 6: return occupancy_nop<6>(ndim, sse);
+    default: return 0;
     }
 }
 
