@@ -12,11 +12,18 @@
  * exactly (built with -ffp-contract=off, no fast-math); the FP64 entry points
  * (suffix _f64) run the same algorithm in double and give the FP64 error.
  *
- * Pinning (see tests/test_oracle.py): FD coefficients against closed forms;
- * index map against hand-computed cases; sponge tables against the closed
- * form; the stencils against brute-force and analytic invariants on tiny grids
- * (constant field, impulse response symmetry, linearity, 1-D reduction); the
- * time ordering by impulse tests (trace row 0 is zero, first arrival index).
+ * Pinning (tests/test_oracle.py; the reference ships no tests or golden data
+ * for this path): FD coefficients against the closed form and the Taylor
+ * moments (optimized tables by their wider accurate wavenumber band); sponge
+ * tables against their closed forms; the index map and the spherical one-way
+ * rule against hand-derived known answers (tests/golden/); every propagator,
+ * FP64 and FP32, against a textbook float64 numpy leapfrog written from the
+ * PDE (cartesian 2*nop-order Laplacian, spherical metric terms, the polar
+ * d/dr sign), both away from and across the absorbing layers (sponge, one-way
+ * faces, free surfaces); linearity, superposition, reciprocity, absorption
+ * against a larger domain and the FP64 error.  Seeded mutations of the
+ * stencil, metric, injection, swap, one-way, sponge, reflectivity, index map
+ * and extraction code each fail at least one of these tests.
  */
 #ifndef AWEFD_ORACLE_H
 #define AWEFD_ORACLE_H
