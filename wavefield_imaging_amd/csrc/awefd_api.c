@@ -192,8 +192,8 @@ int awefd_create(const awefd_params *p, awefd_model **out)
         m->opad[a] = m->p.o[a] - (float)nba * m->p.d[a];
         m->ncell *= m->np[a];
     }
-    /* device rows padded to 128 bytes for the cartesian kernels */
-    m->ld = m->p.kind <= AWEFD_CART3D ? (m->np[0] + 31) / 32 * 32 : m->np[0];
+    /* device rows padded to 128 bytes: a 64-node tile is two whole cache lines */
+    m->ld = (m->np[0] + 31) / 32 * 32;
     m->nalloc = (long)m->ld * m->np[1] * m->np[2];
     if (m->ncell > (1L << 33)) {
         free(m);

@@ -62,6 +62,61 @@ typedef struct {
     float *trow;             /* traces + it*nr                                */
 } awefd_cart_args;
 
+/* Polar (2-D) and spherical (3-D) kernels.  Axes: 1 = r (fastest), 2 = theta,
+ * 3 = phi.  Nodes within AWEFD_SPHER_SHELL layers of a face form the shell:
+ * the step kernel stores their values before the one-way boundary (S) and the
+ * shell kernel finishes them. */
+#define AWEFD_SPHER_SHELL 6
+#define AWEFD_SPHER_NOP 4         /* interior band and one-way depth (spher_utils.c:326) */
+#define AWEFD_SPHER_MAXCHAIN 16   /* longest chain of one-way reads of final values */
+#ifndef AWEFD_SPHER_RX
+#define AWEFD_SPHER_RX 4          /* theta rows per wave of the 3-D step kernel */
+#endif
+#ifndef AWEFD_SPHER_NWAVE
+#define AWEFD_SPHER_NWAVE 4
+#endif
+#define AWEFD_SPHER_TX (AWEFD_SPHER_NWAVE * AWEFD_SPHER_RX)
+
+typedef struct {
+    float       *P;          /* current field (after this step's injection)   */
+    float       *Q;          /* previous field in, new field out (in place)   */
+    float       *S;          /* shell nodes before the one-way boundary       */
+    const float *vv;         /* (dt*dt)*(v*v), padded                         */
+    int n1, n2, n3;          /* padded r, theta, phi (n3 = 1 in 2-D)          */
+    int ld;                  /* row stride in floats                          */
+    int ndim;
+    /* derivative divisors, as the reference forms them in float */
+    float tdr, dr2, tdt, dt2, dp2;  /* 2*dr, dr*dr, 2*dth, dth*dth, dph*dph      */
+    /* 3-D metric: per r 2/r and 1/(r*r) (float); per (theta, r)
+     * cos/((double)(r*r)*sin) and 1/((double)(r*r)*sin*sin) */
+    const float *mr2, *mr1;
+    const double *mt1, *mt2;
+    /* 2-D polar coefficients per r: r-1, centre-r, r+1, theta+-1, centre-theta */
+    const float *pc;         /* 5 floats per r                                */
+    /* sponge: damp[i], i = depth into the strip; fsrf */
+    int nb, fsrf;
+    const double *damp;
+    /* one-way tables: theta faces [phi][r], r faces [phi][theta] (the table
+     * each rule uses), phi faces [theta][r] */
+    const float *bthl, *bthh, *brl, *brh, *bphl, *bphh;
+    /* step tiling */
+    int ntz, ntx, nty, nblk;
+    /* shell enumeration: counts of the r-face, theta-face and phi-face groups */
+    long nsr, nst, nsp;
+    /* injection: nodes (flat offsets) with their contributions in order */
+    int ninode;
+    const long *inode;
+    const int *ibeg;
+    const float *iw;         /* weight (times reflectivity when expl)         */
+    const float *wav;
+    int it;
+    /* extraction */
+    int nr;
+    const int *rj;           /* (jr, jtheta, jphi) per receiver               */
+    const float *rw;
+    float *trow;
+} awefd_spher_args;
+
 /* Launchers (cart_kernels.hip).  Return 0 or a HIP error code. */
 int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndim, int sse, int varden,
                            void *stream);
@@ -69,6 +124,13 @@ int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void *stream);
 int awefd_launch_fill(float *p, long n, float v, void *stream);
 /* resident blocks per CU of the cartesian step kernel (0 if unknown) */
 int awefd_cart_step_occupancy(int nop, int ndim, int sse);
+
+/* Launchers (spher_kernels.hip). */
+int awefd_launch_spher_inject(const awefd_spher_args *a, void *stream);
+int awefd_launch_spher_step(const awefd_spher_args *a, void *stream);
+int awefd_launch_spher_shell(const awefd_spher_args *a, void *stream);
+int awefd_launch_spher_extract(const awefd_spher_args *a, void *stream);
+int awefd_spher_step_occupancy(int ndim);
 
 #ifdef __cplusplus
 }

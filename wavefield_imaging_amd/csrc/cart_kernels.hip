@@ -40,13 +40,14 @@
  * multiply rounds as in the reference's SSE (or scalar) build.
  */
 #include <hip/hip_runtime.h>
+#include "awefd_hip.h"
 #include "awefd_internal.h"
 
 #pragma clang fp contract(off)
 
 namespace {
 
-constexpr int WAVE = 64;
+using namespace awefd;
 constexpr int NWAVE = AWEFD_NWAVE;    /* waves per block */
 constexpr int NTHR = NWAVE * WAVE;
 constexpr int RX = AWEFD_RX;          /* x rows per wave */
@@ -59,46 +60,6 @@ constexpr int RX = AWEFD_RX;          /* x rows per wave */
 constexpr int NT = AWEFD_NT;
 constexpr int TX = NWAVE * RX;
 static_assert(TX == AWEFD_CART_TX, "tile width");
-
-/* Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
- * Give each XCD a contiguous range of tiles so that tiles sharing halos
- * share an L2. */
-__device__ __forceinline__ int xcd_remap(int b, int n)
-{
-    const int per = n >> 3, rem = n & 7, xcd = b & 7, slot = b >> 3;
-    return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
-}
-
-typedef __amdgpu_buffer_rsrc_t rsrc_t;
-constexpr unsigned OOB = 0x80000000u;
-
-__device__ __forceinline__ rsrc_t mk_rsrc(const float *p, unsigned bytes)
-{
-    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
-}
-/* voffset: per-lane bytes; soffset: wave-uniform bytes (both range-checked) */
-template <int POL = 0>
-__device__ __forceinline__ float bld(rsrc_t r, unsigned voff, unsigned soff)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, POL));
-}
-template <int POL = 0>
-__device__ __forceinline__ void bst(rsrc_t r, unsigned voff, unsigned soff, float v)
-{
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)voff, (int)soff, POL);
-}
-/* DPP wave shifts: lane i receives lane i-1 (shr) or lane i+1 (shl); the
- * lane without a source (0 for shr, 63 for shl) keeps `old`. */
-__device__ __forceinline__ float shr1(float old, float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
-                                                                 __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
-}
-__device__ __forceinline__ float shl1(float old, float v)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
-                                                                 __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
-}
 
 template <int NDIM>
 __device__ __forceinline__ float sponge(float v, int iz, int ix, int iy, const awefd_cart_args &a)
