@@ -1,22 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark: plain 3-D cartesian acoustic forward modelling on one MI355X.
+"""Benchmark: the acoustic FD time loop on one MI355X.
 
-Workload: 256 x 512 x 512 (z, x, y) model, h = 5 m, 40-node sponge on every
-face (padded grid 336 x 592 x 592 = 117,755,904 nodes), nt = 2000 steps of
-dt = 6.67e-9 s, three-layer rough-base velocity, one 2 MHz Ricker source,
-128 x 128 receivers, sponge + one-way ABC (dabc=y), FP32.
+Default workload (the headline line): plain 3-D cartesian forward modelling,
+256 x 512 x 512 (z, x, y) model, h = 5 m, 40-node sponge on every face (padded
+grid 336 x 592 x 592 = 117,755,904 nodes), nt = 2000 steps of dt = 6.67e-9 s,
+three-layer rough-base velocity, one 2 MHz Ricker source, 128 x 128 receivers,
+sponge + one-way ABC (dabc=y), FP32.
+
+``--workload spher3d``: the spherical shell, 400 x 512 x 512 (r, theta, phi),
+r from 3387.5 km, dr = 5 m, r*dtheta = 5 m at mid radius, 30-node sponge
+(padded 460 x 572 x 572 = 150,504,640 nodes), nt = 2000, one source 10 nodes
+below the top, 256 x 256 receivers.  The default run also measures it and
+reports it under "also".
 
 One benchmark step = one complete shot (all nt time steps) with every input
 already resident in HBM.  value = node-updates per second over the timed
 steps (Gcell-updates/s).  The roofline figure is for the dominant kernel
 (the fused step kernel): 16 algorithmic bytes per node-update (read the
-previous field, the current field and (v dt)^2, write the new field), times
-the padded nodes per launch, over the kernel's mean duration measured with
-HIP events on the stream it runs on.
+previous field, the current field and the velocity term, write the new
+field), times the padded nodes per launch, over the kernel's mean duration
+measured with HIP events on the stream it runs on.
 
-cpu_baseline: the C oracle (a restatement of the reference CPU program
-sfAWEFDcart3Dcpu, OpenMP over the host cores) on the same padded grid for a
-bounded number of time steps.
+cpu_baseline: the C oracle (a restatement of the reference CPU program,
+OpenMP over the host cores) on the same padded grid for a bounded number of
+time steps.
 """
 import argparse
 import json
@@ -30,10 +37,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = 16
 
+WORKLOADS = {
+    "cart3d": dict(kernel="k_cart_step<2,3,true>",
+                   traffic="cart3d_step_traffic.json",
+                   desc="cart3d 256x512x512 (z,x,y) h=5m nb=40 nt=%d dabc=y fdorder=4, "
+                        "padded 336x592x592, 1 source, 128x128 receivers"),
+    "spher3d": dict(kernel="k_spher_step3",
+                    traffic="spher3d_step_traffic.json",
+                    desc="spher3d 400x512x512 (r,theta,phi) r0=3387.5km dr=5m nb=30 nt=%d "
+                         "expl=n, padded 460x572x572, 1 source, 256x256 receivers"),
+}
 
-def _traffic():
+
+def _traffic(name):
     """HBM bytes per launch of the step kernel from the committed PMC summary."""
-    path = os.path.join(ROOT, "profiles", "cart3d_step_traffic.json")
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     try:
@@ -65,88 +83,96 @@ def cpu_baseline(case, steps):
                           steps, case["par"]["nt"], secs))
 
 
+def measure(workload, nt, steps, warmup):
+    """Time `steps` full shots of `workload` after `warmup` untimed ones."""
+    import numpy as np
+
+    from wavefield_imaging_amd import awefd, synth
+
+    w = WORKLOADS[workload]
+    case = getattr(synth, workload)(nt=nt)
+    m = awefd.Model(workload, case["par"])
+    m.set_model(case["vel"])
+    m.set_sources(case["sou"])
+    m.set_receivers(case["rec"])
+    m.set_wavelet(case["wav"])
+    cells = int(np.prod(m.padded))
+    print("bench: %s padded %s, nt=%d, %d receivers" % (workload, m.padded, nt, m.nr),
+          file=sys.stderr, flush=True)
+    for _ in range(warmup):
+        m.run_dev()
+    m.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m.run_dev()
+    t1 = time.perf_counter()
+    launches, kern_ms = m.kernel_stats()
+    m.profile(False)
+    m.close()
+    secs = t1 - t0
+    avg_ms = kern_ms / max(launches, 1)
+    achieved = BYTES_PER_UPDATE * cells / (avg_ms * 1e-3) / 1e9
+    out = dict(workload=w["desc"] % nt, cells_per_step=cells, time_steps_per_shot=nt,
+               value=round(cells * nt * steps / secs / 1e9, 3),
+               ms_per_step=round(secs / steps * 1e3, 3),
+               roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": _traffic(w["traffic"]), "kernel": w["kernel"],
+                         "kernel_mean_ms": round(avg_ms, 4), "launches": launches})
+    return out, case
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=2000, help="time steps per shot")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="cart3d")
+    ap.add_argument("--also", default="spher3d",
+                    help="comma-separated extra workloads measured after the headline one "
+                         "(one shot each; empty for none)")
     ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
-              "(the time loop is single-GPU; see DESIGN.md for how it would shard)" % args.gpus,
-              file=sys.stderr)
+              "(the time loop runs on a single GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
-
-    from wavefield_imaging_amd import awefd, synth
-
-    case = synth.cart3d(nt=args.nt)
-    par = case["par"]
-    m = awefd.Model("cart3d", par)
-    m.set_model(case["vel"])
-    m.set_sources(case["sou"])
-    m.set_receivers(case["rec"])
-    m.set_wavelet(case["wav"])
-    cells = int(np.prod(m.padded))
-    print("bench: cart3d padded %s, nt=%d, %d receivers" % (m.padded, args.nt, m.nr),
-          file=sys.stderr, flush=True)
-
-    for _ in range(args.warmup):
-        m.run_dev()
-    m.profile(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        m.run_dev()
-    t1 = time.perf_counter()
-    launches, kern_ms = m.kernel_stats()
-    m.profile(False)
-    secs = t1 - t0
-    updates = cells * args.nt * args.steps
-    value = updates / secs / 1e9
-    avg_ms = kern_ms / max(launches, 1)
-    achieved = BYTES_PER_UPDATE * cells / (avg_ms * 1e-3) / 1e9
-    traffic = _traffic()
+    res, case = measure(args.workload, args.nt, args.steps, args.warmup)
     out = {
-        "metric": "Gcell-updates/s (cart3d forward, 1 GPU)",
-        "value": round(value, 3),
+        "metric": "Gcell-updates/s (%s forward, 1 GPU)" % args.workload,
+        "value": res["value"],
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(secs / args.steps * 1e3, 3),
+        "ms_per_step": res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded three-layer rough-base model, Ricker 2 MHz)",
-        "config": {
-            "workload": "cart3d 256x512x512 (z,x,y) h=5m nb=40 nt=%d dabc=y fdorder=4, "
-                        "padded 336x592x592, 1 source, 128x128 receivers" % args.nt,
-            "cells_per_step": cells,
-            "time_steps_per_shot": args.nt,
-        },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": "k_cart_step<2,3,true>",
-            "kernel_mean_ms": round(avg_ms, 4),
-            "launches": launches,
-        },
+        "config": {"workload": res["workload"], "cells_per_step": res["cells_per_step"],
+                   "time_steps_per_shot": res["time_steps_per_shot"]},
+        "roofline": res["roofline"],
     }
-    m.close()
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(case, args.cpu_steps)
         except Exception as e:  # the baseline never hides the GPU line
             out["cpu_baseline"] = {"error": str(e)}
+    del case
+    also = []
+    for wl in [x for x in args.also.split(",") if x and x != args.workload]:
+        try:
+            r, _ = measure(wl, args.nt, 1, 1)
+            also.append(r)
+        except Exception as e:
+            also.append({"workload": wl, "error": str(e)})
+    if also:
+        out["also"] = also
     print(json.dumps(out), flush=True)
 
 

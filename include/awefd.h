@@ -132,7 +132,7 @@ int awefd_debug_points(const awefd_model *m, int which, int *idx, float *w);
 int awefd_profile(awefd_model *m, int enable);
 int awefd_kernel_stats(awefd_model *m, long *launches, double *total_ms);
 
-/* Device the handle runs on, and the raw device pointer of its stream. */
+/* HIP device ordinal the handle runs on. */
 int awefd_device(const awefd_model *m);
 
 const char *awefd_last_error(void);
