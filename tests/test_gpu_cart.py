@@ -29,10 +29,11 @@ def _case3d(nz=24, nx=28, ny=30, nt=60, nb=6, **kw):
     return par, vel, sou, rec, synth.ricker(nt, synth.DT)
 
 
-def _check(kind, par, vel, sou, rec, wav, orc):
-    print("case", kind, par["n"], {k: par[k] for k in par if k not in ("n", "o", "d")}, flush=True)
-    tr, _, field = awefd.run(kind, par, vel, sou, rec, wav)
-    ref, ufin, _ = orc.run(kind, par, vel, sou, rec, wav)
+def _check(kind, par, vel, sou, rec, wav, orc, den=None):
+    print("case", kind, par["n"], {k: par[k] for k in par if k not in ("n", "o", "d")},
+          "den" if den is not None else "", flush=True)
+    tr, _, field = awefd.run(kind, par, vel, sou, rec, wav, den=den)
+    ref, ufin, _ = orc.run(kind, par, vel, sou, rec, wav, den=den)
     assert np.any(ref != 0)
     np.testing.assert_array_equal(tr, ref)
     np.testing.assert_array_equal(field, ufin)
@@ -63,3 +64,31 @@ def test_cart2d_many_sources_separate_wavelets(orc):
                                 dict(sse=False), dict(nb=3)])
 def test_cart3d_bit_exact(orc, kw):
     _check("cart3d", *_case3d(**kw), orc)
+
+
+def _density(vel, seed):
+    """Density loosely tied to the velocity layers plus a smooth perturbation."""
+    rng = np.random.default_rng(seed)
+    d = 900.0 + vel.astype(np.float64) / 2.5e5
+    d *= 1.0 + 0.1 * np.sin(np.cumsum(rng.standard_normal(vel.shape), axis=-1) / 8.0)
+    return d.astype(np.float32)
+
+
+@pytest.mark.parametrize("kind,kw", [
+    ("cart2d", dict()), ("cart2d", dict(sse=False)), ("cart2d", dict(fdorder=8, dabc=False)),
+    ("cart3d", dict()), ("cart3d", dict(sse=False)), ("cart3d", dict(fdorder=6, free=True)),
+])
+def test_variable_density_bit_exact(orc, kind, kw):
+    """cden=n with den= (ts_kernel.c:101-135, 228-280)."""
+    case = _case2d if kind == "cart2d" else _case3d
+    par, vel, sou, rec, wav = case(cden=False, **kw)
+    _check(kind, par, vel, sou, rec, wav, orc, den=_density(vel, 8))
+
+
+def test_constant_density_flag_ignores_den(orc):
+    """cden=y runs constant density even when den= is given."""
+    par, vel, sou, rec, wav = _case2d(cden=True)
+    den = _density(vel, 9)
+    tr, _, _ = awefd.run("cart2d", par, vel, sou, rec, wav, den=den)
+    ref, _, _ = orc.run("cart2d", par, vel, sou, rec, wav)
+    np.testing.assert_array_equal(tr, ref)

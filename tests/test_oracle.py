@@ -251,7 +251,20 @@ def _one_way_faces(un, u, velp, dt, h, nop, nb, free):
             put(i, take(u, i - 1) + (take(u, i) - take(un, i - 1)) * k_hi)
 
 
-def _leapfrog_cart(velp, dt, h, nop, src, recs, wav, nt, nb=0, dabc=False, free=False):
+def _grad_dot(u, rho, h, nop, ndim):
+    """sum over axes of D1(u) D1(rho) with the order-2*nop central first
+    difference, interior nodes only; array axes (.., x, z)."""
+    b = _taylor_d2(nop) * np.arange(1, nop + 1) / 2
+    out = np.zeros_like(u)
+    inner = tuple(slice(nop, n - nop) for n in u.shape)
+    for ax in range(ndim):
+        du = sum(b[k - 1] * (np.roll(u, -k, ax) - np.roll(u, k, ax)) for k in range(1, nop + 1))
+        dr = sum(b[k - 1] * (np.roll(rho, -k, ax) - np.roll(rho, k, ax)) for k in range(1, nop + 1))
+        out[inner] += (du * dr)[inner] / h[ndim - 1 - ax] ** 2
+    return out
+
+
+def _leapfrog_cart(velp, dt, h, nop, src, recs, wav, nt, nb=0, dabc=False, free=False, rho=None):
     """u[n+1] = 2u[n] - u[n-1] + (v dt)^2 lap u[n]; the source adds
     w[n] W (v dt)^2 at each of its nodes; with dabc the 3-D one-way faces
     follow, then the sponge scales u[n+1] and u[n]; trace row n = u[n]
@@ -265,7 +278,10 @@ def _leapfrog_cart(velp, dt, h, nop, src, recs, wav, nt, nb=0, dabc=False, free=
     u = np.zeros_like(V)
     tr = np.zeros((nt, len(recs)))
     for it in range(nt):
-        un = 2 * u - um + V * _lap_cart(u, h, nop, ndim)
+        lap = _lap_cart(u, h, nop, ndim)
+        if rho is not None:   # variable density: lap - grad(rho).grad(u) / rho
+            lap = lap - _grad_dot(u, rho, h, nop, ndim) / rho
+        un = 2 * u - um + V * lap
         for idx, w in src:
             un[idx] += float(wav[it]) * float(w) * V[idx]
         if dabc:
@@ -307,6 +323,39 @@ def test_cartesian_against_textbook_leapfrog(orc, ndim, fdorder, nt):
     # time ordering: row 0 is u[0] = 0; row 1 holds only the injected node
     assert np.all(ref32[0] == 0) and ref32[1, 0] != 0 and np.all(ref32[1, 1:4] == 0)
     assert np.abs(tb[-1]).min() > 0
+    np.testing.assert_allclose(ref64, tb, rtol=1e-9, atol=1e-12 * np.abs(tb).max())
+    np.testing.assert_allclose(ref32, tb, rtol=2e-5, atol=2e-6 * np.abs(tb).max())
+
+
+@pytest.mark.parametrize("ndim,fdorder,sse", [(2, 4, True), (2, 8, False), (3, 4, True), (3, 6, False)])
+def test_cartesian_density_against_textbook(orc, ndim, fdorder, sse):
+    """cden=n: the operator is lap(u) - grad(rho).grad(u) / rho (ts_kernel.c:101-135,
+    228-280), with the first differences of the same order."""
+    nop, nb, nt = fdorder // 2, 6, 7
+    n = [30, 34, 28][:ndim]
+    rng = np.random.default_rng(17)
+    vel = rng.uniform(1.2e8, 3.0e8, size=n[::-1]).astype(np.float32)
+    den = rng.uniform(900.0, 2600.0, size=n[::-1]).astype(np.float32)
+    h = [5.0, 4.0, 6.0][:ndim]
+    si = [15, 17, 14][:ndim]
+    co = lambda z: ([z[1] * h[1], z[0] * h[0]] if ndim == 2 else
+                    [z[1] * h[1], z[2] * h[2], z[0] * h[0]])
+    rz = [si, [si[0] + 2] + si[1:], [si[0] - 1, si[1] + 1.5] + [s + 0.5 for s in si[2:]]]
+    sou = np.array([co(si)], np.float32)
+    rec = np.array([co(r) for r in rz], np.float32)
+    wav = synth.ricker(nt, synth.DT, freq=2e7, delay=0.0)
+    kind = "cart%dd" % ndim
+    par = dict(n=n, o=[0.0] * ndim, d=h, nt=nt, dt=synth.DT, nb=nb, dabc=True, fdorder=fdorder,
+               cden=False, sse=sse)
+    ref32, _, _ = orc.run(kind, par, vel, sou, rec, wav, den=den)
+    ref64, _, _ = orc.run(kind, par, vel, sou, rec, wav, den=den, precision="f64")
+    pt = lambda q: _points(orc, kind, n, par["o"], h, nb, q)
+    rhop = np.pad(den, nb, mode="edge").astype(np.float64)
+    tb, _ = _leapfrog_cart(np.pad(vel, nb, mode="edge"), np.float64(np.float32(synth.DT)), h, nop,
+                           pt(sou)[0], pt(rec), wav.astype(np.float64), nt, rho=rhop)
+    tb0, _ = _leapfrog_cart(np.pad(vel, nb, mode="edge"), np.float64(np.float32(synth.DT)), h, nop,
+                            pt(sou)[0], pt(rec), wav.astype(np.float64), nt)
+    assert np.abs(tb - tb0).max() > 1e-3 * np.abs(tb).max()      # the term matters
     np.testing.assert_allclose(ref64, tb, rtol=1e-9, atol=1e-12 * np.abs(tb).max())
     np.testing.assert_allclose(ref32, tb, rtol=2e-5, atol=2e-6 * np.abs(tb).max())
 

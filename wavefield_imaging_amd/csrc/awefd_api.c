@@ -271,6 +271,51 @@ static int upload(awefd_model *m, void **dst, const void *src, size_t bytes)
     return AWEFD_OK;
 }
 
+/* Variable density (cden=n with den=, MAWEFDcart2Dcpu.c:137-145,264-268): the
+ * padded density and its first differences along z, x, y with the 1st-order
+ * coefficients, as step_forward_2d/3d form them (ts_kernel.c:101-135,228-280).
+ * They do not change with time, so they are computed once here and stored as
+ * one 16-byte record per node {rho, drho_z, drho_x, drho_y}. */
+static int cart_set_density(awefd_model *m, const float *den)
+{
+    const int nbs[3] = {m->nb, m->nb, m->ndim == 3 ? m->nb : 0};
+    const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2], nop = m->nop;
+    float *r = malloc(sizeof(float) * m->ncell);
+    float *g = calloc((size_t)m->ncell * 4, sizeof(float));
+    if (!r || !g) {
+        free(r); free(g);
+        return awefd__fail(AWEFD_ENOMEM, "density buffers");
+    }
+    awefd_pad(den, r, m->p.n, nbs);
+    const long sx = n1, sy = (long)n1 * n2;
+    const float *bz = m->c1, *bx = m->c1 + nop, *by = m->c1 + 2 * nop;
+    for (long i = 0; i < m->ncell; i++) g[4 * i] = r[i];
+    const int y0 = m->ndim == 3 ? nop : 0, y1 = m->ndim == 3 ? n3 - nop : 1;
+    for (int iy = y0; iy < y1; iy++)
+        for (int ix = nop; ix < n2 - nop; ix++)
+            for (int iz = nop; iz < n1 - nop; iz++) {
+                const long i = iy * sy + ix * sx + iz;
+                float dz = 0.f, dx = 0.f, dy = 0.f;
+                for (int k = 1; k <= nop; k++) {
+                    dz = dz + (r[i + k] - r[i - k]) * bz[k];
+                    dx = dx + (r[i + k * sx] - r[i - k * sx]) * bx[k];
+                    if (m->ndim == 3) dy = dy + (r[i + k * sy] - r[i - k * sy]) * by[k];
+                }
+                g[4 * i + 1] = dz;
+                g[4 * i + 2] = dx;
+                g[4 * i + 3] = dy;
+            }
+    free(r);
+    free_dev((void **)&m->drho);
+    hipError_t e = hipMalloc((void **)&m->drho, sizeof(float) * 4 * (size_t)m->nalloc);
+    if (e == hipSuccess)
+        e = hipMemcpy2D(m->drho, sizeof(float) * 4 * m->ld, g, sizeof(float) * 4 * n1, sizeof(float) * 4 * n1,
+                        (size_t)n2 * n3, hipMemcpyHostToDevice);
+    free(g);
+    if (e != hipSuccess) return awefd__fail(AWEFD_EHIP, "density upload: %s", hipGetErrorString(e));
+    return AWEFD_OK;
+}
+
 /* Cartesian model set-up: MAWEFDcart2Dcpu.c:258-295, MAWEFDcart3Dcpu.c:254-293. */
 static int cart_set_model(awefd_model *m, const float *vel, const float *den)
 {
@@ -334,8 +379,8 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
         if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "vel upload: %s", hipGetErrorString(e));
     }
     free(v);
-    if (rc == 0 && den && !m->p.cden)
-        rc = awefd__fail(AWEFD_EINVAL, "variable density (cden=n with den=) is not built yet");
+    if (rc == 0 && den && !m->p.cden) rc = cart_set_density(m, den);
+    else free_dev((void **)&m->drho);     /* constant density */
     return rc;
 }
 
@@ -571,7 +616,7 @@ static void cart_args(const awefd_model *m, awefd_cart_args *a)
 {
     memset(a, 0, sizeof *a);
     a->vel = m->dvel;
-    a->rho = m->drho;
+    a->rho = m->p.cden ? NULL : m->drho;
     a->n1 = m->np[0];
     a->n2 = m->np[1];
     a->n3 = m->np[2];
@@ -656,7 +701,7 @@ static int cart_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, vo
         a.it = it;
         a.trow = dtr ? dtr + (long)it * m->nr : m->d_traces + (long)it * m->nr;
         if (m->prof) HIPCHK(hipEventRecord(m->ev[2 * npairs], m->stream));
-        int e = awefd_launch_cart_step(&a, m->nop, m->ndim, m->p.sse, 0, m->stream);
+        int e = awefd_launch_cart_step(&a, m->nop, m->ndim, m->p.sse, a.rho != NULL, m->stream);
         if (m->prof) HIPCHK(hipEventRecord(m->ev[2 * npairs + 1], m->stream));
         if (m->prof) npairs++;
         if (!e && a.abc) e = awefd_launch_cart_abc_xy(&a, m->nop, m->stream);

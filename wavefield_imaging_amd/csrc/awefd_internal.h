@@ -33,7 +33,8 @@ typedef struct {
     float       *P;          /* previous field, overwritten with the new one  */
     const float *C;          /* current field                                 */
     const float *vel;        /* (v dt)^2, padded                              */
-    const float *rho;        /* density (variable-density runs) or NULL       */
+    const float *rho;        /* variable density: {rho, drho_z, drho_x, drho_y}
+                                per node (ld pitch), else NULL              */
     int n1, n2, n3;          /* padded z, x, y                                */
     int ld;                  /* row stride in floats (n1 rounded up to 32)    */
     int zsep;                /* z-high ABC done by a separate pass            */

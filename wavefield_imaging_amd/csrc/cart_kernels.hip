@@ -137,13 +137,55 @@ __device__ __forceinline__ void load_pvh(float (&p)[RX], float (&v)[RX], float (
     }
 }
 
+/* Variable-density term of ts_kernel.c:101-135 (2-D) and 228-280 (3-D):
+ * lap - (du . drho) / rho, du from the stencil's own neighbour values, drho
+ * precomputed per node (g = {rho, drho_z, drho_x, drho_y}). */
+template <int NOP, int NDIM, bool SSE, int QN>
+__device__ __forceinline__ float den_term(float lap, const float (&zp)[NOP], const float (&zm)[NOP],
+                                          const float (&cen)[RX], const float (&hal)[2 * NOP],
+                                          const float (&qj)[QN], int S0, int j, float4 g,
+                                          const awefd_cart_args &a)
+{
+    float duz = 0.f, dux = 0.f, duy = 0.f;
+#pragma unroll
+    for (int k = 1; k <= NOP; k++) {
+        duz = duz + (zp[k - 1] - zm[k - 1]) * a.bz[k - 1];
+        dux = dux + (xnb<NOP>(cen, hal, j, k) - xnb<NOP>(cen, hal, j, -k)) * a.bx[k - 1];
+        if (NDIM == 3) duy = duy + (qj[(S0 + NOP + k) % QN] - qj[(S0 + NOP - k) % QN]) * a.by[k - 1];
+    }
+    float dd;
+    if (NDIM == 2) dd = duz * g.y + dux * g.z;
+    else if (SSE) dd = duz * g.y + (dux * g.z + duy * g.w);
+    else dd = duz * g.y + dux * g.z + duy * g.w;
+    return lap - dd / g.x;
+}
+
+/* Density record of a node (variable-density runs): rho and its three
+ * time-invariant first differences, four adjacent dwords (the b128 buffer
+ * builtin of this compiler loads only the first dword, so the record is read
+ * as four 32-bit loads, which the compiler merges). */
+__device__ __forceinline__ float4 bld4(rsrc_t r, unsigned voff, unsigned soff)
+{
+    return make_float4(bld<NT>(r, voff, soff), bld<NT>(r, voff + 4u, soff), bld<NT>(r, voff + 8u, soff),
+                       bld<NT>(r, voff + 12u, soff));
+}
+
+template <bool VD>
+__device__ __forceinline__ void load_den(float4 (&g)[RX], rsrc_t rR, unsigned vz4, int x0, int y, int ybase,
+                                         unsigned rowb, unsigned psz)
+{
+    if (!VD) return;
+#pragma unroll
+    for (int j = 0; j < RX; j++) g[j] = bld4(rR, vz4, 4u * soff(x0 + j, y, ybase, rowb, psz));
+}
+
 /* The step kernel.  Planes of the current field live in a ring of
  * QN = 2*NOP+2 register slots; the previous field, (v dt)^2 and the halo
  * values of the next plane in the second of two register buffers; the y loop
  * is unrolled QN times so that every slot index is a compile-time constant.
  * The loads of plane y+1 are in flight while plane y is computed.  No LDS and
  * no barriers: waves advance independently. */
-template <int NOP, int NDIM, bool SSE>
+template <int NOP, int NDIM, bool SSE, bool VD>
 __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
 {
     constexpr int QN = 2 * NOP + 2;
@@ -178,10 +220,14 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
     const rsrc_t rC = mk_rsrc(a.C + ybase * sy, nbytes);
     const rsrc_t rP = mk_rsrc(a.P + ybase * sy, nbytes);
     const rsrc_t rV = mk_rsrc(a.vel + ybase * sy, nbytes);
+    /* density records: 4 floats per node, same layout (window < 4 GiB) */
+    const rsrc_t rR = mk_rsrc(VD ? (const void *)(a.rho + 4 * ybase * sy) : (const void *)a.vel,
+                              VD ? 4u * nbytes : 0u);
 
     /* per-lane offsets: the node itself, and the z value beyond the tile edge
      * this lane fetches (lane k < NOP: z0-1-k; lane 63-k: z0+64+k) */
     const unsigned vz = zin ? (unsigned)iz * 4u : OOB;
+    const unsigned vz4 = zin ? (unsigned)iz * 16u : OOB;   /* density records */
     int zz = -1;
     if (lane < NOP) zz = z0 - 1 - lane;
     else if (lane >= WAVE - NOP) zz = z0 + WAVE + (WAVE - 1 - lane);
@@ -212,6 +258,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
     /* ring: current-field plane p in slot (p - y0 + NOP) % QN */
     float q[RX][QN];
     float bp_[2][RX], bv_[2][RX], bz_[2][RX], bh_[2][2 * NOP];
+    float4 bg_[2][RX];
     if (NDIM == 3) {
 #pragma unroll
         for (int k = 0; k <= 2 * NOP; k++)
@@ -222,6 +269,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
         for (int j = 0; j < RX; j++) q[j][0] = bld(rC, vz, soff(x0 + j, 0, 0, rowb, psz));
     }
     load_pvh<NOP>(bp_[0], bv_[0], bz_[0], bh_[0], rP, rV, rC, vz, vzh, x0, y0, ybase, rowb, psz);
+    load_den<VD>(bg_[0], rR, vz4, x0, y0, ybase, rowb, psz);
 
     /* injection entries of this tile column, restricted to planes >= y0 */
     const int col = tzi + a.ntz * txi;
@@ -249,6 +297,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
                 }
                 load_pvh<NOP>(bp_[B ^ 1], bv_[B ^ 1], bz_[B ^ 1], bh_[B ^ 1], rP, rV, rC, vz, vzh, x0,
                               y + 1, ybase, rowb, psz);
+                load_den<VD>(bg_[B ^ 1], rR, vz4, x0, y + 1, ybase, rowb, psz);
             }
             float cen[RX];
 #pragma unroll
@@ -296,6 +345,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
                         if (NDIM == 3)
                             lap = lap + a.cy[k - 1] * (q[j][(S0 + NOP + k) % QN] + q[j][(S0 + NOP - k) % QN]);
                     }
+                    if (VD) lap = den_term<NOP, NDIM, SSE>(lap, zp, zm, cen, bh_[B], q[j], S0, j, bg_[B][j], a);
                     const float t = 2.f * c - p;
                     v = t + bv_[B][j] * lap;
                 } else {
@@ -307,6 +357,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
                             t = t + (q[j][(S0 + NOP - k) % QN] + q[j][(S0 + NOP + k) % QN]) * a.cy[k - 1];
                         lap = lap + t;
                     }
+                    if (VD) lap = den_term<NOP, NDIM, SSE>(lap, zp, zm, cen, bh_[B], q[j], S0, j, bg_[B][j], a);
                     const float vl = bv_[B][j] * lap;
                     v = (float)(2.0 * (double)c - (double)p + (double)vl);
                 }
@@ -459,10 +510,13 @@ __global__ void k_fill(float *p, long n, float v)
 }
 
 template <int NOP, int NDIM, bool SSE>
-int launch_step(const awefd_cart_args *a, hipStream_t s)
+int launch_step(const awefd_cart_args *a, bool vd, hipStream_t s)
 {
     const int nextract = (a->nr + NTHR - 1) / NTHR;
-    hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE>), dim3(a->nblk + nextract), dim3(NTHR), 0, s, *a);
+    if (vd)
+        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, true>), dim3(a->nblk + nextract), dim3(NTHR), 0, s, *a);
+    else
+        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, false>), dim3(a->nblk + nextract), dim3(NTHR), 0, s, *a);
     return (int)hipGetLastError();
 }
 
@@ -470,7 +524,7 @@ template <int NOP, int NDIM, bool SSE>
 int occupancy_step()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cart_step<NOP, NDIM, SSE>, NTHR, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cart_step<NOP, NDIM, SSE, false>, NTHR, 0) != hipSuccess)
         return 0;
     return nb;
 }
@@ -483,10 +537,10 @@ int occupancy_nop(int ndim, int sse)
 }
 
 template <int NOP>
-int launch_step_nop(const awefd_cart_args *a, int ndim, int sse, hipStream_t s)
+int launch_step_nop(const awefd_cart_args *a, int ndim, int sse, bool vd, hipStream_t s)
 {
-    if (ndim == 2) return sse ? launch_step<NOP, 2, true>(a, s) : launch_step<NOP, 2, false>(a, s);
-    return sse ? launch_step<NOP, 3, true>(a, s) : launch_step<NOP, 3, false>(a, s);
+    if (ndim == 2) return sse ? launch_step<NOP, 2, true>(a, vd, s) : launch_step<NOP, 2, false>(a, vd, s);
+    return sse ? launch_step<NOP, 3, true>(a, vd, s) : launch_step<NOP, 3, false>(a, vd, s);
 }
 
 template <int NOP>
@@ -508,14 +562,15 @@ extern "C" int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndi
                                       int varden, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (varden) return (int)hipErrorInvalidValue;
+    const bool vd = varden != 0;
+    if (vd && !a->rho) return (int)hipErrorInvalidValue;
     switch (nop) {
-    case 1: return launch_step_nop<1>(a, ndim, sse, s);
-    case 2: return launch_step_nop<2>(a, ndim, sse, s);
-    case 3: return launch_step_nop<3>(a, ndim, sse, s);
-    case 4: return launch_step_nop<4>(a, ndim, sse, s);
-    case 5: return launch_step_nop<5>(a, ndim, sse, s);
-    case 6: return launch_step_nop<6>(a, ndim, sse, s);
+    case 1: return launch_step_nop<1>(a, ndim, sse, vd, s);
+    case 2: return launch_step_nop<2>(a, ndim, sse, vd, s);
+    case 3: return launch_step_nop<3>(a, ndim, sse, vd, s);
+    case 4: return launch_step_nop<4>(a, ndim, sse, vd, s);
+    case 5: return launch_step_nop<5>(a, ndim, sse, vd, s);
+    case 6: return launch_step_nop<6>(a, ndim, sse, vd, s);
     default: return (int)hipErrorInvalidValue;
     }
 }
