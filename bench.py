@@ -122,6 +122,52 @@ def measure(workload, nt, steps, warmup):
     return out, case
 
 
+def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
+    """256 independent 2-D shots of uneven size (nz ~ U[200,600], nx ~
+    U[400,1200], nb=50): the batched graph-captured run (autotuned tile) and
+    the plain per-shot loop of the single-grid kernel, in shots/s."""
+    import numpy as np
+
+    from wavefield_imaging_amd import awefd, synth
+
+    bat = synth.batch2d(nshot=nshot, nt=nt)
+    ms = []
+    for sh in bat["shots"]:
+        m = awefd.Model("cart2d", sh["par"])
+        m.set_model(sh["vel"])
+        m.set_sources(sh["sou"])
+        m.set_receivers(sh["rec"])
+        m.set_wavelet(bat["wav"])
+        ms.append(m)
+    cells = int(sum(np.prod(m.padded) for m in ms))
+    print("bench: batch2d %d shots, %.3g padded cells, nt=%d" % (nshot, cells, nt), file=sys.stderr, flush=True)
+    b = awefd.Batch(ms)
+    rx, grp, tune_ms = b.tune(steps=tune_steps)
+    b.run()                                   # warm-up (graph capture + launch)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    t1 = time.perf_counter()
+    batched = nshot * steps / (t1 - t0)
+    out = dict(workload="batch2d %d shots nz~U[200,600] nx~U[400,1200] nb=50 nt=%d, %d padded cells"
+                        % (nshot, nt, cells),
+               shots_per_s=round(batched, 3), gcells_per_s=round(cells * nt * steps / (t1 - t0) / 1e9, 3),
+               s_per_batch=round((t1 - t0) / steps, 4), tile_rx=rx, group_mb=grp,
+               tune_ms_per_step={"rx%d_g%d" % k: round(v, 4) for k, v in tune_ms.items()})
+    b.close()
+    if per_shot:
+        ms[0].run_dev()                       # warm-up
+        t0 = time.perf_counter()
+        for m in ms:
+            m.run_dev()
+        t1 = time.perf_counter()
+        loop = nshot / (t1 - t0)
+        out.update(per_shot_loop_shots_per_s=round(loop, 3), speedup_vs_per_shot_loop=round(batched / loop, 2))
+    for m in ms:
+        m.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,7 +175,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--nt", type=int, default=2000, help="time steps per shot")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="cart3d")
-    ap.add_argument("--also", default="spher3d",
+    ap.add_argument("--also", default="spher3d,batch2d",
                     help="comma-separated extra workloads measured after the headline one "
                          "(one shot each; empty for none)")
     ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline")
@@ -167,8 +213,11 @@ def main():
     also = []
     for wl in [x for x in args.also.split(",") if x and x != args.workload]:
         try:
-            r, _ = measure(wl, args.nt, 1, 1)
-            also.append(r)
+            if wl == "batch2d":
+                also.append(measure_batch(args.nt))
+            else:
+                r, _ = measure(wl, args.nt, 1, 1)
+                also.append(r)
         except Exception as e:
             also.append({"workload": wl, "error": str(e)})
     if also:

@@ -109,6 +109,11 @@ typedef void (*awefd_snap_fn)(void *user, int it, const float *field, const int 
  * cb is called for it % jsnap == 0 when jsnap > 0 (cb may be NULL). */
 int awefd_run(awefd_model *m, float *traces, int jsnap, awefd_snap_fn cb, void *user);
 
+/* Traces of the last run (nt*nr floats, row it = receivers at step it) from
+ * device memory; used after awefd_run_dev or a batched run.  Replaces the
+ * per-step sf_floatwrite of the traces (MAWEFDcart2Dcpu.c:373-378). */
+int awefd_get_traces(awefd_model *m, float *host);
+
 /* Same loop with every buffer already resident in HBM: d_traces is a device
  * pointer to nt*nr floats (may be NULL); no host traffic, no snapshots. */
 int awefd_run_dev(awefd_model *m, float *d_traces);
@@ -137,6 +142,31 @@ int awefd_device(const awefd_model *m);
 
 const char *awefd_last_error(void);
 void awefd_destroy(awefd_model *m);
+
+/* ---- Batched shots.  No counterpart in the reference, which runs one
+ * program per shot (byrne_awe.py:163-250): for callers with many independent
+ * 2-D cartesian shots of uneven size.  The shots' time loops run together,
+ * one launch per step over every shot, captured into one HIP graph. ---- */
+typedef struct awefd_batch awefd_batch;
+
+/* Group fully set-up AWEFD_CART2D models (same nt, fdorder, sse and device;
+ * constant density).  The models keep their buffers and must outlive the
+ * batch. */
+int awefd_batch_create(int nshot, awefd_model *const *models, awefd_batch **out);
+/* Tile shape (x rows per wave: 2, 4, 8 or 16; default 8) and shot grouping:
+ * group_mb > 0 runs consecutive shots whose fields total at most group_mb MiB
+ * through all their steps before the next group (cache-resident groups);
+ * 0 advances every shot in every launch. */
+int awefd_batch_set_tile(awefd_batch *b, int rx, int group_mb);
+/* Autotune both: time `steps` steps of every candidate (rx in 2, 4, 8, 16 x
+ * group_mb in 0, 32, 64, 128, 192) on the batch and keep the fastest.
+ * Outputs are nullable; ms_out[4*5] gets ms per step, row-major by rx. */
+int awefd_batch_tune(awefd_batch *b, int steps, int *rx_out, int *group_mb_out, double *ms_out);
+/* Run every shot's whole time loop (fields reset first).  Afterwards each
+ * model holds what its own awefd_run_dev would leave: traces via
+ * awefd_get_traces, the field via awefd_get_field. */
+int awefd_batch_run(awefd_batch *b);
+void awefd_batch_destroy(awefd_batch *b);
 
 #ifdef __cplusplus
 }

@@ -21,7 +21,8 @@ EXPORTS = (
     "awefd_set_sources", "awefd_set_receivers", "awefd_set_wavelet", "awefd_run",
     "awefd_run_dev", "awefd_reset", "awefd_get_field", "awefd_debug_points",
     "awefd_profile", "awefd_kernel_stats", "awefd_device", "awefd_last_error",
-    "awefd_destroy",
+    "awefd_destroy", "awefd_get_traces", "awefd_batch_create", "awefd_batch_set_tile",
+    "awefd_batch_tune", "awefd_batch_run", "awefd_batch_destroy",
 )
 
 
@@ -70,6 +71,12 @@ def load():
         "awefd_device": (C.c_int, [C.c_void_p]),
         "awefd_last_error": (C.c_char_p, []),
         "awefd_destroy": (None, [C.c_void_p]),
+        "awefd_get_traces": (C.c_int, [C.c_void_p, P(C.c_float)]),
+        "awefd_batch_create": (C.c_int, [C.c_int, P(C.c_void_p), P(C.c_void_p)]),
+        "awefd_batch_set_tile": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "awefd_batch_tune": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double)]),
+        "awefd_batch_run": (C.c_int, [C.c_void_p]),
+        "awefd_batch_destroy": (None, [C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

@@ -119,6 +119,12 @@ class Model:
         """Full time loop with device-resident output (pointer or None)."""
         L.check(self.lib.awefd_run_dev(self.h, C.c_void_p(d_traces_ptr) if d_traces_ptr else None))
 
+    def traces(self):
+        """Traces of the last run, read back from device memory."""
+        tr = np.zeros((self.nt, max(self.nr, 0)), np.float32)
+        L.check(self.lib.awefd_get_traces(self.h, L.fptr(tr)))
+        return tr
+
     def field(self):
         """Padded field the reference would write after the last step."""
         out = np.zeros(self.padded[: self.ndim][::-1], np.float32)
@@ -157,6 +163,48 @@ class Model:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Batch:
+    """Many 2-D cartesian shots advanced together (one launch per step over
+    every shot, the time loop captured into one HIP graph)."""
+
+    def __init__(self, models):
+        self.lib = L.load()
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m.h.value for m in self.models])
+        h = C.c_void_p()
+        L.check(self.lib.awefd_batch_create(len(self.models), arr, C.byref(h)))
+        self.h = h
+
+    RX = (2, 4, 8, 16)
+    GROUP_MB = (0, 32, 64, 128, 192)
+
+    def set_tile(self, rx, group_mb=0):
+        L.check(self.lib.awefd_batch_set_tile(self.h, int(rx), int(group_mb)))
+
+    def tune(self, steps=20):
+        """Time every (tile, group) candidate on the batch and keep the fastest;
+        returns (rx, group_mb, {(rx, group_mb): ms per step})."""
+        rx, grp = C.c_int(), C.c_int()
+        ms = (C.c_double * (len(self.RX) * len(self.GROUP_MB)))()
+        L.check(self.lib.awefd_batch_tune(self.h, int(steps), C.byref(rx), C.byref(grp), ms))
+        keys = [(r, g) for r in self.RX for g in self.GROUP_MB]
+        return rx.value, grp.value, dict(zip(keys, list(ms)))
+
+    def run(self):
+        L.check(self.lib.awefd_batch_run(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.awefd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def run(kind, par, vel, sou, rec, wav, den=None, snapshots=False):

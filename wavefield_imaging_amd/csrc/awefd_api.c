@@ -437,15 +437,22 @@ static int set_points(awefd_model *m, int which, int n, const float *coords)
 }
 
 /* Injection buckets of the cartesian step kernel (one per z-tile/x-tile
- * column), holding every (node, source, weight) contribution. */
-static int cart_build_inj(awefd_model *m)
+ * column of a tile tz x tx), holding every (node, source, weight)
+ * contribution; host arrays, freed by the caller. */
+int awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj **inj, int **ibeg_out,
+                        int *nbucket_out, long *ninj)
 {
     const int nc = m->ndim == 3 ? 8 : 4;
     const long n = (long)m->ns * nc;
-    const int nbucket = m->ntz * m->ntx;
+    const int ntx = (m->np[1] + tx - 1) / tx;
+    const int nbucket = ntz * ntx;
     awefd_inj_key *keys = malloc(sizeof(*keys) * (n > 0 ? n : 1));
     awefd_inj *e = malloc(sizeof(*e) * (n > 0 ? n : 1));
     int *ibeg = malloc(sizeof(int) * (nbucket + 1));
+    if (!keys || !e || !ibeg) {
+        free(keys); free(e); free(ibeg);
+        return awefd__fail(AWEFD_ENOMEM, "injection buckets");
+    }
     for (int s = 0; s < m->ns; s++)
         for (int q = 0; q < nc; q++) {
             awefd_inj_key *k = &keys[(long)s * nc + q];
@@ -454,13 +461,28 @@ static int cart_build_inj(awefd_model *m)
             k->e.y = m->ndim == 3 ? m->h_sj[3 * s + 2] + ((q >> 2) & 1) : 0;
             k->e.src = s;
             k->e.w = m->h_sw[8 * s + q];
-            k->bucket = (k->e.z / m->tz) + m->ntz * (k->e.x / AWEFD_CART_TX);
+            k->bucket = (k->e.z / tz) + ntz * (k->e.x / tx);
         }
     awefd_sort_inj(keys, n, nbucket, e, ibeg);
-    int rc = upload(m, (void **)&m->d_inj, e, sizeof(*e) * (n > 0 ? n : 1));
+    free(keys);
+    *inj = e;
+    *ibeg_out = ibeg;
+    *nbucket_out = nbucket;
+    *ninj = n;
+    return AWEFD_OK;
+}
+
+static int cart_build_inj(awefd_model *m)
+{
+    awefd_inj *e;
+    int *ibeg, nbucket;
+    long n;
+    int rc = awefd__cart_buckets(m, m->tz, m->ntz, AWEFD_CART_TX, &e, &ibeg, &nbucket, &n);
+    if (rc) return rc;
+    rc = upload(m, (void **)&m->d_inj, e, sizeof(*e) * (n > 0 ? n : 1));
     if (!rc) rc = upload(m, (void **)&m->d_ibeg, ibeg, sizeof(int) * (nbucket + 1));
     m->ninj = n;
-    free(keys); free(e); free(ibeg);
+    free(e); free(ibeg);
     return rc;
 }
 
@@ -612,7 +634,7 @@ int awefd_get_field(awefd_model *m, float *host)
     return AWEFD_OK;
 }
 
-static void cart_args(const awefd_model *m, awefd_cart_args *a)
+void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
 {
     memset(a, 0, sizeof *a);
     a->vel = m->dvel;
@@ -675,7 +697,7 @@ static int check_ready(awefd_model *m)
     return AWEFD_OK;
 }
 
-static int ensure_traces(awefd_model *m)
+int awefd__ensure_traces(awefd_model *m)
 {
     const long n = (long)m->p.nt * m->nr;
     if (m->ntraces >= n && m->d_traces) return AWEFD_OK;
@@ -689,7 +711,7 @@ static int ensure_traces(awefd_model *m)
 static int cart_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, void *user)
 {
     awefd_cart_args a;
-    cart_args(m, &a);
+    awefd__cart_args(m, &a);
     const int nt = m->p.nt;
     float *P = m->dP, *C = m->dC;
     float *host = NULL;
@@ -730,7 +752,7 @@ static int run_common(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, v
     int rc = check_ready(m);
     if (rc) return rc;
     HIPCHK(hipSetDevice(m->dev));
-    if ((rc = ensure_traces(m))) return rc;
+    if ((rc = awefd__ensure_traces(m))) return rc;
     if ((rc = awefd_reset(m))) return rc;
     if (is_cart(m)) rc = cart_loop(m, dtr, jsnap, cb, user);
     else rc = awefd__spher_loop(m, dtr, jsnap, cb, user);
@@ -747,6 +769,17 @@ int awefd_run(awefd_model *m, float *traces, int jsnap, awefd_snap_fn cb, void *
     if (traces && m->nr > 0)
         HIPCHK(hipMemcpy(traces, m->d_traces, sizeof(float) * (size_t)m->p.nt * m->nr,
                          hipMemcpyDeviceToHost));
+    return AWEFD_OK;
+}
+
+int awefd_get_traces(awefd_model *m, float *host)
+{
+    if (!m || !host) return awefd__fail(AWEFD_EINVAL, "awefd_get_traces: NULL argument");
+    if (!m->d_traces) return awefd__fail(AWEFD_ESTATE, "no traces yet");
+    HIPCHK(hipSetDevice(m->dev));
+    HIPCHK(hipStreamSynchronize(m->stream));
+    if (m->nr > 0)
+        HIPCHK(hipMemcpy(host, m->d_traces, sizeof(float) * (size_t)m->p.nt * m->nr, hipMemcpyDeviceToHost));
     return AWEFD_OK;
 }
 

@@ -20,6 +20,7 @@ extern "C" {
 #define AWEFD_NWAVE 4             /* waves per block of the cartesian step kernel */
 #endif
 #define AWEFD_CART_TX (AWEFD_NWAVE * AWEFD_RX) /* x rows per stencil block     */
+#define AWEFD_BATCH_NWAVE 4      /* waves per block of the batched 2-D kernel     */
 
 /* One source-injection contribution: node (z, x, y) += amp(src) * w * vel. */
 typedef struct {
@@ -60,7 +61,8 @@ typedef struct {
     int nr;
     const int *rj;           /* 3 ints per receiver (jz, jx, jy)              */
     const float *rw;         /* 8 weights per receiver (4 used in 2-D)       */
-    float *trow;             /* traces + it*nr                                */
+    float *trow;             /* traces + it*nr (batched: the traces base)     */
+    int blk0;                /* batched: first block of this shot             */
 } awefd_cart_args;
 
 /* Polar (2-D) and spherical (3-D) kernels.  Axes: 1 = r (fastest), 2 = theta,
@@ -123,6 +125,10 @@ int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndim, int sse,
                            void *stream);
 int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void *stream);
 int awefd_launch_fill(float *p, long n, float v, void *stream);
+/* Batched 2-D step over many shots (cart_kernels.hip): shots = device array
+ * of per-shot arguments, bshot = shot of every block, rx in {2,4,8,16}. */
+int awefd_launch_cart_batch2(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int nop,
+                             int sse, int rx, void *stream);
 /* resident blocks per CU of the cartesian step kernel (0 if unknown) */
 int awefd_cart_step_occupancy(int nop, int ndim, int sse);
 

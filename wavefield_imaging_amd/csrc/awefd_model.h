@@ -68,6 +68,12 @@ struct awefd_model {
 
 int awefd__fail(int code, const char *fmt, ...);
 
+/* cartesian helpers shared with the batched path (awefd_api.c) */
+void awefd__cart_args(const awefd_model *m, awefd_cart_args *a);
+int  awefd__ensure_traces(awefd_model *m);
+int  awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj **inj, int **ibeg,
+                         int *nbucket, long *ninj);
+
 /* spherical back end (spher_host.c) */
 int  awefd__spher_create(awefd_model *m);
 int  awefd__spher_set_model(awefd_model *m, const float *vel);

@@ -48,9 +48,9 @@
 namespace {
 
 using namespace awefd;
-constexpr int NWAVE = AWEFD_NWAVE;    /* waves per block */
-constexpr int NTHR = NWAVE * WAVE;
-constexpr int RX = AWEFD_RX;          /* x rows per wave */
+constexpr int NWAVE0 = AWEFD_NWAVE;   /* waves per block of the single-grid kernel */
+constexpr int NTHR0 = NWAVE0 * WAVE;
+constexpr int RX0 = AWEFD_RX;         /* x rows per wave of the single-grid kernel */
 /* cache policy of the streamed arrays (previous field, (v dt)^2, stores):
  * non-temporal, so that L2 keeps the current field's rows for their reuse
  * as halos */
@@ -58,8 +58,7 @@ constexpr int RX = AWEFD_RX;          /* x rows per wave */
 #define AWEFD_NT 2
 #endif
 constexpr int NT = AWEFD_NT;
-constexpr int TX = NWAVE * RX;
-static_assert(TX == AWEFD_CART_TX, "tile width");
+static_assert(NWAVE0 * RX0 == AWEFD_CART_TX, "tile width");
 
 template <int NDIM>
 __device__ __forceinline__ float sponge(float v, int iz, int ix, int iy, const awefd_cart_args &a)
@@ -81,7 +80,7 @@ __device__ __forceinline__ bool in_sponge(int iz, int ix, int iy, const awefd_ca
 /* lint2d_extract / lint3d_extract of the reference's u0, i.e. the current
  * field with its pending sponge factor. */
 template <int NDIM>
-__device__ void cart_extract(const awefd_cart_args &a, int r)
+__device__ void cart_extract(const awefd_cart_args &a, const float *C, float *trow, int r)
 {
     if (r >= a.nr) return;
     const long sx = a.ld, sy = (long)a.ld * a.n2;
@@ -91,18 +90,18 @@ __device__ void cart_extract(const awefd_cart_args &a, int r)
 #pragma unroll
     for (int q = 0; q < NC; q++) {
         const int z = jz + (q & 1), x = jx + ((q >> 1) & 1), y = NDIM == 3 ? jy + ((q >> 2) & 1) : 0;
-        float u = a.C[y * sy + x * sx + z];
+        float u = C[y * sy + x * sx + z];
         if (a.dabc) u = sponge<NDIM>(u, z, x, y, a);
         const float t = u * a.rw[8 * r + q];
         s = q == 0 ? t : s + t;
     }
-    a.trow[r] = s;
+    trow[r] = s;
 }
 
 /* x neighbour at offset `off` of row j of a wave: its own rows, else the
  * halo rows (h < NOP: x0-NOP+h, h >= NOP: x0+RX+h-NOP).  j and off are
  * compile-time constants after unrolling. */
-template <int NOP>
+template <int NOP, int RX>
 __device__ __forceinline__ float xnb(const float (&cen)[RX], const float (&hal)[2 * NOP], int j, int off)
 {
     const int r = j + off;
@@ -117,7 +116,7 @@ __device__ __forceinline__ unsigned soff(int x, int y, int ybase, unsigned rowb,
     return (unsigned)x * rowb + (unsigned)(y - ybase) * psz;
 }
 
-template <int NOP>
+template <int NOP, int RX>
 __device__ __forceinline__ void load_pvh(float (&p)[RX], float (&v)[RX], float (&zh)[RX],
                                          float (&hal)[2 * NOP], rsrc_t rP, rsrc_t rV, rsrc_t rC,
                                          unsigned vz, unsigned vzh, int x0, int y, int ybase,
@@ -140,7 +139,7 @@ __device__ __forceinline__ void load_pvh(float (&p)[RX], float (&v)[RX], float (
 /* Variable-density term of ts_kernel.c:101-135 (2-D) and 228-280 (3-D):
  * lap - (du . drho) / rho, du from the stencil's own neighbour values, drho
  * precomputed per node (g = {rho, drho_z, drho_x, drho_y}). */
-template <int NOP, int NDIM, bool SSE, int QN>
+template <int NOP, int NDIM, bool SSE, int QN, int RX>
 __device__ __forceinline__ float den_term(float lap, const float (&zp)[NOP], const float (&zm)[NOP],
                                           const float (&cen)[RX], const float (&hal)[2 * NOP],
                                           const float (&qj)[QN], int S0, int j, float4 g,
@@ -150,7 +149,7 @@ __device__ __forceinline__ float den_term(float lap, const float (&zp)[NOP], con
 #pragma unroll
     for (int k = 1; k <= NOP; k++) {
         duz = duz + (zp[k - 1] - zm[k - 1]) * a.bz[k - 1];
-        dux = dux + (xnb<NOP>(cen, hal, j, k) - xnb<NOP>(cen, hal, j, -k)) * a.bx[k - 1];
+        dux = dux + (xnb<NOP, RX>(cen, hal, j, k) - xnb<NOP, RX>(cen, hal, j, -k)) * a.bx[k - 1];
         if (NDIM == 3) duy = duy + (qj[(S0 + NOP + k) % QN] - qj[(S0 + NOP - k) % QN]) * a.by[k - 1];
     }
     float dd;
@@ -170,7 +169,7 @@ __device__ __forceinline__ float4 bld4(rsrc_t r, unsigned voff, unsigned soff)
                        bld<NT>(r, voff + 12u, soff));
 }
 
-template <bool VD>
+template <bool VD, int RX>
 __device__ __forceinline__ void load_den(float4 (&g)[RX], rsrc_t rR, unsigned vz4, int x0, int y, int ybase,
                                          unsigned rowb, unsigned psz)
 {
@@ -185,16 +184,18 @@ __device__ __forceinline__ void load_den(float4 (&g)[RX], rsrc_t rR, unsigned vz
  * is unrolled QN times so that every slot index is a compile-time constant.
  * The loads of plane y+1 are in flight while plane y is computed.  No LDS and
  * no barriers: waves advance independently. */
-template <int NOP, int NDIM, bool SSE, bool VD>
-__global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
+template <int NOP, int NDIM, bool SSE, bool VD, int RX, int NW>
+__device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *Pout, const float *Cin, int it,
+                                               float *trow, int bid)
 {
     constexpr int QN = 2 * NOP + 2;
+    constexpr int TX = NW * RX;
     const int tid = threadIdx.x;
-    if ((int)blockIdx.x >= a.nblk) {
-        cart_extract<NDIM>(a, ((int)blockIdx.x - a.nblk) * NTHR + tid);
+    if (bid >= a.nblk) {
+        cart_extract<NDIM>(a, Cin, trow, (bid - a.nblk) * (NW * WAVE) + tid);
         return;
     }
-    const int lb = xcd_remap(blockIdx.x, a.nblk);
+    const int lb = xcd_remap(bid, a.nblk);
     const int tzi = lb % a.ntz;
     const int rr = lb / a.ntz;
     const int txi = rr % a.ntx;
@@ -217,8 +218,8 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
     const int ybase = NDIM == 3 ? max(y0 - NOP, 0) : 0;
     const int ytop = NDIM == 3 ? min(y1 + NOP, n3) : 1;
     const unsigned nbytes = (unsigned)(ytop - ybase) * psz;
-    const rsrc_t rC = mk_rsrc(a.C + ybase * sy, nbytes);
-    const rsrc_t rP = mk_rsrc(a.P + ybase * sy, nbytes);
+    const rsrc_t rC = mk_rsrc(Cin + ybase * sy, nbytes);
+    const rsrc_t rP = mk_rsrc(Pout + ybase * sy, nbytes);
     const rsrc_t rV = mk_rsrc(a.vel + ybase * sy, nbytes);
     /* density records: 4 floats per node, same layout (window < 4 GiB) */
     const rsrc_t rR = mk_rsrc(VD ? (const void *)(a.rho + 4 * ybase * sy) : (const void *)a.vel,
@@ -268,8 +269,8 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
 #pragma unroll
         for (int j = 0; j < RX; j++) q[j][0] = bld(rC, vz, soff(x0 + j, 0, 0, rowb, psz));
     }
-    load_pvh<NOP>(bp_[0], bv_[0], bz_[0], bh_[0], rP, rV, rC, vz, vzh, x0, y0, ybase, rowb, psz);
-    load_den<VD>(bg_[0], rR, vz4, x0, y0, ybase, rowb, psz);
+    load_pvh<NOP, RX>(bp_[0], bv_[0], bz_[0], bh_[0], rP, rV, rC, vz, vzh, x0, y0, ybase, rowb, psz);
+    load_den<VD, RX>(bg_[0], rR, vz4, x0, y0, ybase, rowb, psz);
 
     /* injection entries of this tile column, restricted to planes >= y0 */
     const int col = tzi + a.ntz * txi;
@@ -295,9 +296,9 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
                     for (int j = 0; j < RX; j++)
                         q[j][(S0 + 2 * NOP + 1) % QN] = bld(rC, vz, soff(x0 + j, y + 1 + NOP, ybase, rowb, psz));
                 }
-                load_pvh<NOP>(bp_[B ^ 1], bv_[B ^ 1], bz_[B ^ 1], bh_[B ^ 1], rP, rV, rC, vz, vzh, x0,
+                load_pvh<NOP, RX>(bp_[B ^ 1], bv_[B ^ 1], bz_[B ^ 1], bh_[B ^ 1], rP, rV, rC, vz, vzh, x0,
                               y + 1, ybase, rowb, psz);
-                load_den<VD>(bg_[B ^ 1], rR, vz4, x0, y + 1, ybase, rowb, psz);
+                load_den<VD, RX>(bg_[B ^ 1], rR, vz4, x0, y + 1, ybase, rowb, psz);
             }
             float cen[RX];
 #pragma unroll
@@ -339,7 +340,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
                 if (SSE) {
 #pragma unroll
                     for (int k = 1; k <= NOP; k++) {
-                        const float xpk = xnb<NOP>(cen, bh_[B], j, k), xmk = xnb<NOP>(cen, bh_[B], j, -k);
+                        const float xpk = xnb<NOP, RX>(cen, bh_[B], j, k), xmk = xnb<NOP, RX>(cen, bh_[B], j, -k);
                         lap = lap + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]);
                         lap = lap + a.cx[k - 1] * (xpk + xmk);
                         if (NDIM == 3)
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
                 } else {
 #pragma unroll
                     for (int k = 1; k <= NOP; k++) {
-                        const float xpk = xnb<NOP>(cen, bh_[B], j, k), xmk = xnb<NOP>(cen, bh_[B], j, -k);
+                        const float xpk = xnb<NOP, RX>(cen, bh_[B], j, k), xmk = xnb<NOP, RX>(cen, bh_[B], j, -k);
                         float t = (zm[k - 1] + zp[k - 1]) * a.cz[k - 1] + (xmk + xpk) * a.cx[k - 1];
                         if (NDIM == 3)
                             t = t + (q[j][(S0 + NOP - k) % QN] + q[j][(S0 + NOP + k) % QN]) * a.cy[k - 1];
@@ -368,7 +369,7 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
             int e = ib;
             for (; e < ie && a.inj[e].y == y; e++) {
                 const awefd_inj en = a.inj[e];
-                const float amp = a.wav[(long)a.it * a.wstride + (a.wexpl ? 0 : en.src)];
+                const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
 #pragma unroll
                 for (int j = 0; j < RX; j++) {
                     const float inj = nv[j] + amp * en.w * bv_[B][j];
@@ -423,6 +424,32 @@ __global__ __launch_bounds__(NTHR) void k_cart_step(const awefd_cart_args a)
             }
         }
     }
+}
+
+/* The single-grid step kernel: one launch per time step of one model. */
+template <int NOP, int NDIM, bool SSE, bool VD>
+__global__ __launch_bounds__(NTHR0) void k_cart_step(const awefd_cart_args a)
+{
+    cart_step_body<NOP, NDIM, SSE, VD, RX0, NWAVE0>(a, a.P, a.C, a.it, a.trow, blockIdx.x);
+}
+
+/* The batched 2-D step kernel: one launch advances every shot of a batch one
+ * time step.  Each shot's arguments live in device memory; block b belongs
+ * to shot bshot[b] and is that shot's block b - blk0 (stencil tiles, then
+ * its extraction blocks).  The two field buffers of a shot alternate roles
+ * with the parity of the step, and its trace row is traces + it*nr.  Tile
+ * shape (RX rows per wave, NW waves per block) is a template parameter, chosen
+ * per batch by the host autotuner. */
+template <int NOP, bool SSE, int RX, int NW>
+__global__ __launch_bounds__(NW * WAVE) void k_cart_batch2(const awefd_cart_args *__restrict__ shots,
+                                                           const int *__restrict__ bshot, int it)
+{
+    const int s = __builtin_amdgcn_readfirstlane(bshot[blockIdx.x]);
+    const awefd_cart_args &a = shots[s];
+    const bool odd = it & 1;
+    float *P = odd ? (float *)a.C : a.P;
+    const float *C = odd ? a.P : a.C;
+    cart_step_body<NOP, 2, SSE, false, RX, NW>(a, P, C, it, a.trow + (long)it * a.nr, (int)blockIdx.x - a.blk0);
 }
 
 /* z-high one-way ABC as a separate pass, for grids whose last z tile is too
@@ -512,11 +539,11 @@ __global__ void k_fill(float *p, long n, float v)
 template <int NOP, int NDIM, bool SSE>
 int launch_step(const awefd_cart_args *a, bool vd, hipStream_t s)
 {
-    const int nextract = (a->nr + NTHR - 1) / NTHR;
+    const int nextract = (a->nr + NTHR0 - 1) / NTHR0;
     if (vd)
-        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, true>), dim3(a->nblk + nextract), dim3(NTHR), 0, s, *a);
+        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, true>), dim3(a->nblk + nextract), dim3(NTHR0), 0, s, *a);
     else
-        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, false>), dim3(a->nblk + nextract), dim3(NTHR), 0, s, *a);
+        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, false>), dim3(a->nblk + nextract), dim3(NTHR0), 0, s, *a);
     return (int)hipGetLastError();
 }
 
@@ -524,7 +551,7 @@ template <int NOP, int NDIM, bool SSE>
 int occupancy_step()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cart_step<NOP, NDIM, SSE, false>, NTHR, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cart_step<NOP, NDIM, SSE, false>, NTHR0, 0) != hipSuccess)
         return 0;
     return nb;
 }
@@ -556,7 +583,43 @@ int launch_abc(const awefd_cart_args *a, hipStream_t s)
     return (int)hipGetLastError();
 }
 
+template <int NOP, bool SSE>
+int launch_batch_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int rx, hipStream_t s)
+{
+    constexpr int NW = AWEFD_BATCH_NWAVE;
+    switch (rx) {
+    case 2: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 2, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    case 4: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 4, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    case 8: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 8, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    case 16: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 16, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+template <int NOP>
+int launch_batch_nop(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int sse, int rx, hipStream_t s)
+{
+    return sse ? launch_batch_sse<NOP, true>(shots, bshot, nblk, it, rx, s)
+               : launch_batch_sse<NOP, false>(shots, bshot, nblk, it, rx, s);
+}
+
 } // namespace
+
+extern "C" int awefd_launch_cart_batch2(const awefd_cart_args *shots, const int *bshot, int nblk, int it,
+                                        int nop, int sse, int rx, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    switch (nop) {
+    case 1: return launch_batch_nop<1>(shots, bshot, nblk, it, sse, rx, s);
+    case 2: return launch_batch_nop<2>(shots, bshot, nblk, it, sse, rx, s);
+    case 3: return launch_batch_nop<3>(shots, bshot, nblk, it, sse, rx, s);
+    case 4: return launch_batch_nop<4>(shots, bshot, nblk, it, sse, rx, s);
+    case 5: return launch_batch_nop<5>(shots, bshot, nblk, it, sse, rx, s);
+    case 6: return launch_batch_nop<6>(shots, bshot, nblk, it, sse, rx, s);
+    default: return (int)hipErrorInvalidValue;
+    }
+}
 
 extern "C" int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndim, int sse,
                                       int varden, void *stream)
