@@ -168,6 +168,27 @@ int awefd_batch_tune(awefd_batch *b, int steps, int *rx_out, int *group_mb_out, 
 int awefd_batch_run(awefd_batch *b);
 void awefd_batch_destroy(awefd_batch *b);
 
+/* ---- Misfit gradient.  No counterpart in the reference, whose adj=y mode
+ * (MAWEFDcart2Dcpu.c:305-316) back-propagates data but forms no gradient:
+ * for inversion callers that need dJ/ds of J(s) = sum r * d(s) over the
+ * traces d of a cartesian 2-D loop, s = 1/v^2 on the model grid. ---- */
+
+/* The forward loop of awefd_run_dev (same traces, same final fields) that
+ * also keeps the field pair of every ckpt-th step (<= 0: ceil(sqrt(nt))) in
+ * device memory for awefd_adjoint_grad: 2 ceil(nt/ckpt) padded fields.
+ * AWEFD_CART2D with constant density only. */
+int awefd_run_ckpt(awefd_model *m, float *d_traces, int ckpt);
+/* dJ/ds at the model of the last awefd_run_ckpt (AWEFD_ESTATE if a setter
+ * ran since).  d_res: device float[nt][nr], the trace weights r = dJ/dd (for
+ * an L2 misfit the residual d - d_obs).  d_grad: device float[n2][n1] (n1
+ * fastest).  Recomputes each segment of fields from its checkpoint; holds
+ * ckpt + 4 more padded fields (kept for later calls until
+ * awefd_adjoint_release or awefd_destroy).  The model's fields keep the
+ * forward run's final state. */
+int awefd_adjoint_grad(awefd_model *m, const float *d_res, float *d_grad);
+/* Free the checkpoint and adjoint buffers of a model. */
+void awefd_adjoint_release(awefd_model *m);
+
 #ifdef __cplusplus
 }
 #endif

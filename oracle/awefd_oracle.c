@@ -281,6 +281,86 @@ int orc_spher3d_rule(int ix, int iz, int iy, int nxpad, int nzpad, int nypad)
 }
 
 /* ------------------------------------------------------------------ */
+/* Set-up of the FP64 cartesian 2-D scheme for the adjoint oracle
+ * (oracle_adj.c): the same padding, coefficients, sponge, (v dt)^2 and index
+ * map as orc_cart2d_f64 (MAWEFDcart2Dcpu.c:209-295). */
+static int orc_fdcoef_d(int nop, int ndim, const float *d, int optfd, int order, double *out);
+
+void orc_cart2d_free_f64(orc_cart2d_f64_t *g)
+{
+    free(g->G); free(g->V); free(g->dVds); free(g->snode); free(g->rnode); free(g->sw); free(g->rw);
+    memset(g, 0, sizeof *g);
+}
+
+int orc_cart2d_setup_f64(const orc_par *p, const float *vel_in, const double *slow, const float *sou,
+                         const float *rec, orc_cart2d_f64_t *g)
+{
+    memset(g, 0, sizeof *g);
+    const int fdorder = p->fdorder > 0 ? p->fdorder : 4;
+    const int nop = fdorder / 2;
+    if (fdorder % 2 || nop < 1 || nop > 6) return -2;
+    int nb = p->nb < nop ? nop : p->nb;
+    g->nop = nop; g->nb = nb; g->nt = p->nt; g->ns = p->ns; g->nr = p->nr; g->expl = p->expl;
+    g->mz = p->n[0]; g->mx = p->n[1];
+    g->nz = g->mz + 2 * nb; g->nx = g->mx + 2 * nb;
+    g->wstride = p->expl ? 1 : p->ns;
+    const long N = (long)g->nz * g->nx;
+    const float d[2] = {p->d[0], p->d[1]};
+    if (orc_fdcoef_d(nop, 2, d, p->optfd, 2, g->c2)) return -2;
+    g->G = malloc(sizeof(double) * N); g->V = malloc(sizeof(double) * N); g->dVds = malloc(sizeof(double) * N);
+    g->snode = malloc(sizeof(long) * 4 * (p->ns + 1)); g->rnode = malloc(sizeof(long) * 4 * (p->nr + 1));
+    g->sw = malloc(sizeof(double) * 4 * (p->ns + 1)); g->rw = malloc(sizeof(double) * 4 * (p->nr + 1));
+    if (!g->G || !g->V || !g->dVds || !g->snode || !g->rnode || !g->sw || !g->rw) {
+        orc_cart2d_free_f64(g);
+        return -3;
+    }
+    double spo[4096];
+    if (nb > 4096) { orc_cart2d_free_f64(g); return -3; }
+    for (int i = 0; i < nb; i++) {
+        double fb = i / (sqrt(2.0) * 4.0 * nb);
+        spo[i] = exp(-fb * fb);
+    }
+    const double dt = p->dt;
+    for (int ix = 0; ix < g->nx; ix++) {
+        int kx = ix - nb; kx = kx < 0 ? 0 : (kx >= g->mx ? g->mx - 1 : kx);
+        for (int iz = 0; iz < g->nz; iz++) {
+            int kz = iz - nb; kz = kz < 0 ? 0 : (kz >= g->mz ? g->mz - 1 : kz);
+            const long i = (long)ix * g->nz + iz;
+            const double v = slow ? 1.0 / sqrt(slow[(long)kx * g->mz + kz]) : vel_in[(long)kx * g->mz + kz];
+            /* free surface: V = 0 on the top nb+1 rows (MAWEFDcart2Dcpu.c:289-295) */
+            g->V[i] = p->fsrf && iz <= nb ? 0.0 : v * v * dt * dt;
+            g->dVds[i] = -g->V[i] * v * v;
+            double gz = 1.0, gx = 1.0;
+            if (p->dabc) {
+                if (iz < nb) gz = spo[nb - 1 - iz];
+                else if (iz >= g->nz - nb) gz = spo[nb - 1 - (g->nz - 1 - iz)];
+                if (ix < nb) gx = spo[nb - 1 - ix];
+                else if (ix >= g->nx - nb) gx = spo[nb - 1 - (g->nx - 1 - ix)];
+            }
+            g->G[i] = gz * gx;
+        }
+    }
+    const float ozp = p->o[0] - (float)nb * p->d[0], oxp = p->o[1] - (float)nb * p->d[1];
+    int *jz = malloc(sizeof(int) * (p->ns + p->nr + 1)), *jx = malloc(sizeof(int) * (p->ns + p->nr + 1));
+    float *w = malloc(sizeof(float) * 4 * (p->ns + p->nr + 1));
+    if (!jz || !jx || !w) { free(jz); free(jx); free(w); orc_cart2d_free_f64(g); return -3; }
+    orc_lint2d(p->ns, sou, g->nz, g->nx, ozp, oxp, p->d[0], p->d[1], jz, jx, w);
+    for (int s = 0; s < p->ns; s++)
+        for (int q = 0; q < 4; q++) {
+            g->snode[4 * s + q] = (long)(jx[s] + ((q >> 1) & 1)) * g->nz + jz[s] + (q & 1);
+            g->sw[4 * s + q] = w[4 * s + q];
+        }
+    orc_lint2d(p->nr, rec, g->nz, g->nx, ozp, oxp, p->d[0], p->d[1], jz, jx, w);
+    for (int r = 0; r < p->nr; r++)
+        for (int q = 0; q < 4; q++) {
+            g->rnode[4 * r + q] = (long)(jx[r] + ((q >> 1) & 1)) * g->nz + jz[r] + (q & 1);
+            g->rw[4 * r + q] = w[4 * r + q];
+        }
+    free(jz); free(jx); free(w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* Propagators, instantiated for float and double. */
 #define REAL float
 #define SFX(name) name##_f32

@@ -113,6 +113,32 @@ double orc_spher_damp(int i, int nb);
  * reference's serial sweep applies at that cell. */
 int  orc_spher3d_rule(int ir, int ith, int iph, int nrpad, int nthpad, int nphpad);
 
+/* ---- FP64 tangent-linear model, adjoint and misfit gradient of the
+ * cartesian 2-D loop (oracle_adj.c).  Constant density, no free surface.
+ * Squared slowness s = 1/v^2 on the model grid (mz x mx, z fastest). ---- */
+typedef struct {
+    int nz, nx, mz, mx, nb, nop, nt, ns, nr, expl;
+    long wstride;
+    double c2[19];
+    double *G, *V, *dVds;        /* padded: sponge product, (v dt)^2, dV/ds      */
+    long *snode, *rnode;         /* 4 nodes per point                            */
+    double *sw, *rw;
+} orc_cart2d_f64_t;
+
+/* The model is vel (float) or, when slow != NULL, the squared slowness slow
+ * (double, model grid).  Gradient: J = 1/2 sum r^2 with r = res_in when given,
+ * else d - dobs (dobs NULL: zero); dJ/ds into grad[mz*mx], J into *misfit,
+ * forward traces into dat_out[nt*nr] (nullable). */
+int orc_cart2d_grad_f64(const orc_par *p, const float *vel, const double *slow, const float *sou,
+                        const float *rec, const float *wav, const float *dobs, const double *res_in,
+                        double *grad, double *misfit, double *dat_out);
+/* traces dd[nt*nr] of the model perturbation ds[mz*mx] (tangent-linear) */
+int orc_cart2d_tlm_f64(const orc_par *p, const float *vel, const double *slow, const float *sou,
+                       const float *rec, const float *wav, const double *ds, double *dd);
+/* traces of the same FP64 scheme */
+int orc_cart2d_fwd_f64(const orc_par *p, const float *vel, const double *slow, const float *sou,
+                       const float *rec, const float *wav, double *dat);
+
 #ifdef __cplusplus
 }
 #endif

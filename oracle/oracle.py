@@ -176,3 +176,73 @@ def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, sn
     if rc:
         raise ValueError("oracle rejected the configuration (rc=%d)" % rc)
     return dat, (ufin.reshape(shp[::-1]) if final else None), shots
+
+
+def _par2d(par, sou, rec):
+    p = OrcPar()
+    for i in range(3):
+        p.n[i] = int((list(par["n"]) + [1])[i])
+        p.o[i] = float((list(par.get("o", [0.0, 0.0])) + [0.0])[i])
+        p.d[i] = float((list(par["d"]) + [1.0])[i])
+    p.nt = int(par["nt"])
+    p.dt = float(par["dt"])
+    p.nb = int(par.get("nb", -1) if par.get("nb") is not None else -1)
+    p.fdorder = int(par.get("fdorder", 4))
+    p.optfd = int(bool(par.get("optfd", False)))
+    p.dabc = int(bool(par.get("dabc", False)))
+    p.fsrf = int(bool(par.get("free", par.get("fsrf", False))))
+    p.expl = int(bool(par.get("expl", False)))
+    p.ns = np.asarray(sou).size // 2
+    p.nr = np.asarray(rec).size // 2
+    return p
+
+
+def _f(a):
+    return np.ascontiguousarray(a, np.float32).reshape(-1)
+
+
+def grad_f64(par, vel, sou, rec, wav, dobs=None, res=None, slow=None):
+    """FP64 L2-misfit gradient w.r.t. squared slowness (model grid): returns
+    (grad, misfit, traces).  res, when given, replaces d - dobs (the adjoint
+    source); slow (float64, model grid) replaces vel as the model."""
+    p = _par2d(par, sou, rec)
+    vel = np.ascontiguousarray(vel, np.float32)
+    g = np.zeros(vel.shape, np.float64)
+    J = C.c_double()
+    dat = np.zeros((p.nt, p.nr), np.float64)
+    do = None if dobs is None else _f(dobs)
+    rs = None if res is None else np.ascontiguousarray(res, np.float64)
+    sl = None if slow is None else np.ascontiguousarray(slow, np.float64)
+    rc = lib().orc_cart2d_grad_f64(C.byref(p), _fp(vel), _dp(sl) if sl is not None else None, _fp(_f(sou)),
+                                   _fp(_f(rec)), _fp(_f(wav)), _fp(do) if do is not None else None,
+                                   _dp(rs) if rs is not None else None, _dp(g), C.byref(J), _dp(dat))
+    if rc:
+        raise ValueError("oracle gradient rejected the configuration (rc=%d)" % rc)
+    return g, J.value, dat
+
+
+def tlm_f64(par, vel, sou, rec, wav, ds, slow=None):
+    """FP64 traces of the squared-slowness perturbation ds (model grid)."""
+    p = _par2d(par, sou, rec)
+    vel = np.ascontiguousarray(vel, np.float32)
+    ds = np.ascontiguousarray(ds, np.float64)
+    sl = None if slow is None else np.ascontiguousarray(slow, np.float64)
+    dd = np.zeros((p.nt, p.nr), np.float64)
+    rc = lib().orc_cart2d_tlm_f64(C.byref(p), _fp(vel), _dp(sl) if sl is not None else None, _fp(_f(sou)),
+                                  _fp(_f(rec)), _fp(_f(wav)), _dp(ds), _dp(dd))
+    if rc:
+        raise ValueError("oracle TLM rejected the configuration (rc=%d)" % rc)
+    return dd
+
+
+def fwd_f64(par, vel, sou, rec, wav, slow=None):
+    """Traces of the FP64 scheme the adjoint oracle differentiates."""
+    p = _par2d(par, sou, rec)
+    sl = None if slow is None else np.ascontiguousarray(slow, np.float64)
+    dat = np.zeros((p.nt, p.nr), np.float64)
+    rc = lib().orc_cart2d_fwd_f64(C.byref(p), _fp(np.ascontiguousarray(vel, np.float32)),
+                                  _dp(sl) if sl is not None else None, _fp(_f(sou)), _fp(_f(rec)),
+                                  _fp(_f(wav)), _dp(dat))
+    if rc:
+        raise ValueError("oracle forward rejected the configuration (rc=%d)" % rc)
+    return dat

@@ -95,6 +95,7 @@ void awefd_destroy(awefd_model *m)
     free(m->h_sp[0]); free(m->h_sp[1]); free(m->h_sp[2]);
     free(m->h_srefl);
     awefd__spher_free(m);
+    awefd__adjoint_free(m);
     free(m);
 }
 
@@ -390,6 +391,7 @@ int awefd_set_model(awefd_model *m, const float *vel, const float *den)
     HIPCHK(hipSetDevice(m->dev));
     int rc = is_cart(m) ? cart_set_model(m, vel, den) : awefd__spher_set_model(m, vel);
     if (rc == 0) m->model_set = 1;
+    m->gen++;
     return rc;
 }
 
@@ -493,6 +495,7 @@ int awefd_set_sources(awefd_model *m, int ns, const float *coords)
     HIPCHK(hipSetDevice(m->dev));
     rc = is_cart(m) ? cart_build_inj(m) : awefd__spher_set_sources(m);
     if (!rc) m->src_set = 1;
+    m->gen++;
     return rc;
 }
 
@@ -504,6 +507,7 @@ int awefd_set_receivers(awefd_model *m, int nr, const float *coords)
     rc = upload(m, (void **)&m->d_rj, m->h_rj, sizeof(int) * 3 * (nr > 0 ? nr : 1));
     if (!rc) rc = upload(m, (void **)&m->d_rw, m->h_rw, sizeof(float) * 8 * (nr > 0 ? nr : 1));
     if (!rc) m->rec_set = 1;
+    m->gen++;
     return rc;
 }
 
@@ -516,6 +520,7 @@ int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav)
         m->nwav = nsamples;
         m->wav_set = 1;
     }
+    m->gen++;
     return rc;
 }
 

@@ -33,6 +33,8 @@ typedef struct {
 typedef struct {
     float       *P;          /* previous field, overwritten with the new one  */
     const float *C;          /* current field                                 */
+    float       *Pn;         /* 2-D only: new field out, P left as it was
+                                (NULL: in place over P)                       */
     const float *vel;        /* (v dt)^2, padded                              */
     const float *rho;        /* variable density: {rho, drho_z, drho_x, drho_y}
                                 per node (ld pitch), else NULL              */
@@ -64,6 +66,36 @@ typedef struct {
     float *trow;             /* traces + it*nr (batched: the traces base)     */
     int blk0;                /* batched: first block of this shot             */
 } awefd_cart_args;
+
+/* Adjoint of the cartesian 2-D loop (adj_kernels.hip). */
+typedef struct {
+    const float *A1, *B1;    /* alpha_{n+1}, beta_{n+1}                       */
+    float *A0, *B0;          /* alpha_n, beta_n                               */
+    const float *Cn;         /* c_n: the current field at the start of step n */
+    const float *V;          /* (v dt)^2                                      */
+    float *gV;               /* dJ/dV accumulator (padded grid)               */
+    int nz, nx, ld, mz, mx, nb, dabc;
+    float idt2;              /* 1/dt^2                                        */
+    const float *sp1, *sp2;  /* sponge factors per axis                       */
+    float c0, cz[AWEFD_MAX_NOP], cx[AWEFD_MAX_NOP];
+    /* receivers: per node, (receiver, weight) in receiver order */
+    int nrnode;
+    const long *rnode;
+    const int *rbeg, *rid;
+    const float *rw;
+    const float *res;        /* residual row n (nr floats)                    */
+    /* injection nodes: per node, (wavelet column, weight) */
+    int nsnode;
+    const long *snode;
+    const int *sbeg, *sid;
+    const float *sw;
+    const float *wav;
+    long wstride;
+    int it;
+} awefd_adj2d_args;
+
+int awefd_launch_adj2d_step(const awefd_adj2d_args *a, int nop, void *stream);
+int awefd_launch_adj2d_chain(const awefd_adj2d_args *a, float *gs, void *stream);
 
 /* Polar (2-D) and spherical (3-D) kernels.  Axes: 1 = r (fastest), 2 = theta,
  * 3 = phi.  Nodes within AWEFD_SPHER_SHELL layers of a face form the shell:

@@ -64,6 +64,11 @@ struct awefd_model {
     int nev;
 
     int steps_done;
+
+    /* gradient state (awefd_adjoint.c): checkpoints of the last
+     * awefd_run_ckpt, valid while gen is unchanged; setters bump gen */
+    void *adj;
+    unsigned gen;
 };
 
 int awefd__fail(int code, const char *fmt, ...);
@@ -73,6 +78,8 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a);
 int  awefd__ensure_traces(awefd_model *m);
 int  awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj **inj, int **ibeg,
                          int *nbucket, long *ninj);
+
+void awefd__adjoint_free(awefd_model *m);
 
 /* spherical back end (spher_host.c) */
 int  awefd__spher_create(awefd_model *m);

@@ -186,7 +186,7 @@ __device__ __forceinline__ void load_den(float4 (&g)[RX], rsrc_t rR, unsigned vz
  * no barriers: waves advance independently. */
 template <int NOP, int NDIM, bool SSE, bool VD, int RX, int NW>
 __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *Pout, const float *Cin, int it,
-                                               float *trow, int bid)
+                                               float *trow, int bid, float *Pdst)
 {
     constexpr int QN = 2 * NOP + 2;
     constexpr int TX = NW * RX;
@@ -220,6 +220,7 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
     const unsigned nbytes = (unsigned)(ytop - ybase) * psz;
     const rsrc_t rC = mk_rsrc(Cin + ybase * sy, nbytes);
     const rsrc_t rP = mk_rsrc(Pout + ybase * sy, nbytes);
+    const rsrc_t rN = mk_rsrc(Pdst + ybase * sy, nbytes);
     const rsrc_t rV = mk_rsrc(a.vel + ybase * sy, nbytes);
     /* density records: 4 floats per node, same layout (window < 4 GiB) */
     const rsrc_t rR = mk_rsrc(VD ? (const void *)(a.rho + 4 * ybase * sy) : (const void *)a.vel,
@@ -420,7 +421,7 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
                     if (NDIM == 3) w = w * spy;
                     v = zdef ? v : w;
                 }
-                bst<NT>(rP, vz, soff(x0 + j, y, ybase, rowb, psz), v);
+                bst<NT>(rN, vz, soff(x0 + j, y, ybase, rowb, psz), v);
             }
         }
     }
@@ -430,7 +431,8 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
 template <int NOP, int NDIM, bool SSE, bool VD>
 __global__ __launch_bounds__(NTHR0) void k_cart_step(const awefd_cart_args a)
 {
-    cart_step_body<NOP, NDIM, SSE, VD, RX0, NWAVE0>(a, a.P, a.C, a.it, a.trow, blockIdx.x);
+    float *dst = NDIM == 2 && a.Pn ? a.Pn : a.P;
+    cart_step_body<NOP, NDIM, SSE, VD, RX0, NWAVE0>(a, a.P, a.C, a.it, a.trow, blockIdx.x, dst);
 }
 
 /* The batched 2-D step kernel: one launch advances every shot of a batch one
@@ -449,7 +451,7 @@ __global__ __launch_bounds__(NW * WAVE) void k_cart_batch2(const awefd_cart_args
     const bool odd = it & 1;
     float *P = odd ? (float *)a.C : a.P;
     const float *C = odd ? a.P : a.C;
-    cart_step_body<NOP, 2, SSE, false, RX, NW>(a, P, C, it, a.trow + (long)it * a.nr, (int)blockIdx.x - a.blk0);
+    cart_step_body<NOP, 2, SSE, false, RX, NW>(a, P, C, it, a.trow + (long)it * a.nr, (int)blockIdx.x - a.blk0, P);
 }
 
 /* z-high one-way ABC as a separate pass, for grids whose last z tile is too
@@ -627,6 +629,7 @@ extern "C" int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndi
     hipStream_t s = (hipStream_t)stream;
     const bool vd = varden != 0;
     if (vd && !a->rho) return (int)hipErrorInvalidValue;
+    if (a->Pn && ndim != 2) return (int)hipErrorInvalidValue;
     switch (nop) {
     case 1: return launch_step_nop<1>(a, ndim, sse, vd, s);
     case 2: return launch_step_nop<2>(a, ndim, sse, vd, s);
