@@ -168,6 +168,110 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
     return out
 
 
+def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0):
+    """Misfit gradients w.r.t. squared slowness, 1000 x 2000 grid, nb=50,
+    nt=4000, checkpoints every 64 steps, 2000 receivers per shot: the per-shot
+    loop (forward with checkpoints, residual, adjoint) timed on `sample`
+    shots, and the batched graphs (every shot's checkpointed forward in one
+    graph, residuals, every shot's backward pass in one graph summing the
+    gradient), in gradients/s."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from wavefield_imaging_amd import _lib as L
+    from wavefield_imaging_amd import awefd, synth
+
+    case = synth.adjoint2d(nt=nt, nshot=nshot)
+    par, rec, wav, K = case["par"], case["rec"], case["wav"], case["ckpt"]
+    dev = torch.device("cuda", 0)
+    nr = len(rec)
+    mt = awefd.Model("cart2d", par)
+    mt.set_model(case["true"])
+    mt.set_receivers(rec)
+    mt.set_wavelet(wav)
+    cells = int(np.prod(mt.padded))
+    print("bench: adjoint2d padded %s, nt=%d, %d shots, %d receivers" % (mt.padded, nt, nshot, nr),
+          file=sys.stderr, flush=True)
+    nobs = nshot if batched else sample
+    dobs = torch.empty((nobs, nt, nr), dtype=torch.float32, device=dev)
+    for s in range(nobs):
+        mt.set_sources(case["shots"][s])
+        torch.cuda.synchronize()
+        L.check(mt.lib.awefd_run_dev(mt.h, C.c_void_p(dobs[s].data_ptr())))
+    mt.close()
+    m = awefd.Model("cart2d", par)
+    m.set_model(case["start"])
+    m.set_receivers(rec)
+    m.set_wavelet(wav)
+    tr = torch.empty((nt, nr), dtype=torch.float32, device=dev)
+    g = torch.empty((par["n"][1], par["n"][0]), dtype=torch.float32, device=dev)
+    gsum = torch.zeros_like(g)
+
+    def one(s):
+        m.set_sources(case["shots"][s])
+        L.check(m.lib.awefd_run_ckpt(m.h, C.c_void_p(tr.data_ptr()), K))
+        res = tr - dobs[s]
+        torch.cuda.current_stream().synchronize()
+        L.check(m.lib.awefd_adjoint_grad(m.h, C.c_void_p(res.data_ptr()), C.c_void_p(g.data_ptr())))
+        gsum.add_(g)
+
+    one(0)                                     # warm-up: buffers, lists
+    gsum.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(sample):
+        one(s)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loop = sample / (t1 - t0)
+    out = dict(workload="adjoint2d 1000x2000 nb=50 nt=%d ckpt=%d, %d shots x %d receivers, padded %s"
+                        % (nt, K, nshot, nr, "x".join(map(str, m.padded[:2]))),
+               per_shot_loop_gradients_per_s=round(loop, 3),
+               per_shot_loop_sample="%d of %d shots, %.2f s" % (sample, nshot, t1 - t0),
+               per_shot_gcells_per_s_per_propagation=round(3 * cells * nt * sample / (t1 - t0) / 1e9, 3))
+    m.close()
+    del tr, g, gsum
+    if batched:
+        ms = []
+        for s in range(nshot):
+            mm = awefd.Model("cart2d", par)
+            mm.set_model(case["start"])
+            mm.set_sources(case["shots"][s])
+            mm.set_receivers(rec)
+            mm.set_wavelet(wav)
+            ms.append(mm)
+        b = awefd.Batch(ms)
+        b.set_tile(rx, group_mb)
+        trs = [mm.traces_tensor() for mm in ms]
+        res = [torch.empty((nt, nr), dtype=torch.float32, device=dev) for _ in range(nshot)]
+        gb = torch.empty((par["n"][1], par["n"][0]), dtype=torch.float32, device=dev)
+
+        def batch_once():
+            b.run_ckpt(K)
+            for s in range(nshot):
+                torch.sub(trs[s], dobs[s], out=res[s])
+            torch.cuda.current_stream().synchronize()
+            b.adjoint_grad([r.data_ptr() for r in res], gb.data_ptr())
+
+        batch_once()                           # warm-up: buffers, graph capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        batch_once()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        bat = nshot / (t1 - t0)
+        out.update(batched_gradients_per_s=round(bat, 3), batched_s_per_gradient_sum=round(t1 - t0, 3),
+                   batched_tile_rx=rx, batched_group_mb=group_mb,
+                   batched_gcells_per_s_per_propagation=round(3 * cells * nt * nshot / (t1 - t0) / 1e9, 3),
+                   speedup_vs_per_shot_loop=round(bat / loop, 2))
+        b.close()
+        for mm in ms:
+            mm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +319,8 @@ def main():
         try:
             if wl == "batch2d":
                 also.append(measure_batch(args.nt))
+            elif wl == "adjoint2d":
+                also.append(measure_grad())
             else:
                 r, _ = measure(wl, args.nt, 1, 1)
                 also.append(r)

@@ -118,6 +118,11 @@ int awefd_get_traces(awefd_model *m, float *host);
  * pointer to nt*nr floats (may be NULL); no host traffic, no snapshots. */
 int awefd_run_dev(awefd_model *m, float *d_traces);
 
+/* Device address of the handle's own trace buffer (nt*nr floats): the traces
+ * of the last run that wrote there (awefd_run, awefd_run_dev without a
+ * buffer, batched runs).  Valid until awefd_destroy. */
+int awefd_traces_dev(awefd_model *m, float **d_traces);
+
 /* Reset the wavefields to zero (the state before the first step). */
 int awefd_reset(awefd_model *m);
 
@@ -167,6 +172,16 @@ int awefd_batch_tune(awefd_batch *b, int steps, int *rx_out, int *group_mb_out, 
  * awefd_get_traces, the field via awefd_get_field. */
 int awefd_batch_run(awefd_batch *b);
 void awefd_batch_destroy(awefd_batch *b);
+/* The batch's awefd_run_ckpt: every shot's forward loop with checkpoints
+ * every ckpt steps (<= 0: ceil(sqrt(nt))), one graph.  Traces as after
+ * awefd_batch_run. */
+int awefd_batch_run_ckpt(awefd_batch *b, int ckpt);
+/* Sum over the shots of awefd_adjoint_grad, every shot's backward pass in
+ * one graph (one launch per step over a group of shots).  d_res: nshot
+ * device pointers (float[nt][nr] of each shot); d_grad: device
+ * float[n2][n1].  The shots must share the grid (n, nb) and come from the
+ * last awefd_batch_run_ckpt; the sum runs in shot order. */
+int awefd_batch_adjoint_grad(awefd_batch *b, const float *const *d_res, float *d_grad);
 
 /* ---- Misfit gradient.  No counterpart in the reference, whose adj=y mode
  * (MAWEFDcart2Dcpu.c:305-316) back-propagates data but forms no gradient:
@@ -182,7 +197,7 @@ int awefd_run_ckpt(awefd_model *m, float *d_traces, int ckpt);
  * ran since).  d_res: device float[nt][nr], the trace weights r = dJ/dd (for
  * an L2 misfit the residual d - d_obs).  d_grad: device float[n2][n1] (n1
  * fastest).  Recomputes each segment of fields from its checkpoint; holds
- * ckpt + 4 more padded fields (kept for later calls until
+ * ckpt + 3 more padded fields (kept for later calls until
  * awefd_adjoint_release or awefd_destroy).  The model's fields keep the
  * forward run's final state. */
 int awefd_adjoint_grad(awefd_model *m, const float *d_res, float *d_grad);

@@ -219,3 +219,48 @@ def test_gradient_errors(orc):
     with pytest.raises(L.AwefdError, match="2-D"):
         _run_ckpt(m3)
     m3.close()
+
+
+def test_batched_gradient_is_the_sum_of_single_shots(orc):
+    """Five shots on one grid (sources at different columns): the batched
+    forward + backward graphs give each shot's traces and the sum of the
+    single-shot gradients bit for bit (the sum in shot order), with one
+    group and with several groups, on repeated launches; and the sum agrees
+    with the FP64 oracle's."""
+    par, vel, sou, rec, wav = _case(nz=100, nx=150, nt=157, nb=8)
+    xs = [60.0, 180.0, 330.0, 470.0, 610.0]
+    rng = np.random.default_rng(4)
+    ms, resl, single_g, single_tr = [], [], [], []
+    for x in xs:
+        m = _model(par, vel, np.float32([[x, 45.0]]), rec, wav)
+        ms.append(m)
+        resl.append(rng.standard_normal((par["nt"], len(rec))).astype(np.float32))
+    for m, r in zip(ms, resl):
+        single_tr.append(_run_ckpt(m, 10).cpu().numpy())
+        single_g.append(_grad(m, r))
+    gsum = single_g[0].copy()
+    for g in single_g[1:]:
+        gsum = gsum + g
+    b = awefd.Batch(ms)
+    dres = [_dev(r) for r in resl]
+    out = torch.empty(gsum.shape, dtype=torch.float32, device="cuda")
+    for rx, grp in ((8, 0), (4, 1)):          # 1 MiB groups: two groups of these shots
+        b.set_tile(rx, grp)
+        for rep in range(2):
+            print("batched gradient rx %d group %d replay %d" % (rx, grp, rep), flush=True)
+            b.run_ckpt(10)
+            for s, m in enumerate(ms):
+                np.testing.assert_array_equal(m.traces(), single_tr[s], err_msg="shot %d" % s)
+            out.fill_(0.0)
+            torch.cuda.synchronize()
+            b.adjoint_grad([t.data_ptr() for t in dres], out.data_ptr())
+            np.testing.assert_array_equal(out.cpu().numpy(), gsum)
+    g64 = None
+    for x, r in zip(xs, resl):
+        g, _, _ = orc.grad_f64(par, vel, np.float32([[x, 45.0]]), rec, wav, res=r)
+        g64 = g if g64 is None else g64 + g
+    print("  batched sum vs FP64 oracle %.3e" % _rel(gsum, g64), flush=True)
+    assert _rel(gsum, g64) < GRAD_RTOL
+    b.close()
+    for m in ms:
+        m.close()

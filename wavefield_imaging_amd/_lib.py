@@ -23,7 +23,8 @@ EXPORTS = (
     "awefd_profile", "awefd_kernel_stats", "awefd_device", "awefd_last_error",
     "awefd_destroy", "awefd_get_traces", "awefd_batch_create", "awefd_batch_set_tile",
     "awefd_batch_tune", "awefd_batch_run", "awefd_batch_destroy", "awefd_run_ckpt",
-    "awefd_adjoint_grad", "awefd_adjoint_release",
+    "awefd_adjoint_grad", "awefd_adjoint_release", "awefd_batch_run_ckpt", "awefd_batch_adjoint_grad",
+    "awefd_traces_dev",
 )
 
 
@@ -81,6 +82,9 @@ def load():
         "awefd_run_ckpt": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int]),
         "awefd_adjoint_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
         "awefd_adjoint_release": (None, [C.c_void_p]),
+        "awefd_batch_run_ckpt": (C.c_int, [C.c_void_p, C.c_int]),
+        "awefd_traces_dev": (C.c_int, [C.c_void_p, P(C.c_void_p)]),
+        "awefd_batch_adjoint_grad": (C.c_int, [C.c_void_p, P(C.c_void_p), C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

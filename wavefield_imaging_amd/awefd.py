@@ -61,6 +61,14 @@ def make_params(kind, par):
     return p
 
 
+class _DevArray:
+    """A float32 device buffer described by __cuda_array_interface__."""
+
+    def __init__(self, ptr, shape):
+        self.__cuda_array_interface__ = {"shape": tuple(shape), "typestr": "<f4", "data": (int(ptr), False),
+                                         "version": 3, "strides": None}
+
+
 class Model:
     """One grid resident on one GPU (an awefd_model handle)."""
 
@@ -124,6 +132,19 @@ class Model:
         tr = np.zeros((self.nt, max(self.nr, 0)), np.float32)
         L.check(self.lib.awefd_get_traces(self.h, L.fptr(tr)))
         return tr
+
+    def traces_dev(self):
+        """Device address of the handle's trace buffer (nt x nr float32)."""
+        p = C.c_void_p()
+        L.check(self.lib.awefd_traces_dev(self.h, C.byref(p)))
+        return p.value
+
+    def traces_tensor(self):
+        """The handle's trace buffer as a torch tensor sharing its memory
+        (no copy); its contents change with the next run."""
+        import torch
+
+        return torch.as_tensor(_DevArray(self.traces_dev(), (self.nt, self.nr)), device="cuda")
 
     def field(self):
         """Padded field the reference would write after the last step."""
@@ -194,6 +215,17 @@ class Batch:
 
     def run(self):
         L.check(self.lib.awefd_batch_run(self.h))
+
+    def run_ckpt(self, ckpt=0):
+        """Every shot's forward loop keeping checkpoints every ``ckpt`` steps
+        (0: ceil(sqrt(nt))) for adjoint_grad."""
+        L.check(self.lib.awefd_batch_run_ckpt(self.h, int(ckpt)))
+
+    def adjoint_grad(self, res_ptrs, grad_ptr):
+        """Sum over the shots of dJ/ds (device pointer grad_ptr, model grid)
+        for per-shot residuals at device pointers res_ptrs (nt x nr each)."""
+        arr = (C.c_void_p * len(self.models))(*[int(p) for p in res_ptrs])
+        L.check(self.lib.awefd_batch_adjoint_grad(self.h, arr, C.c_void_p(int(grad_ptr))))
 
     def close(self):
         if getattr(self, "h", None):

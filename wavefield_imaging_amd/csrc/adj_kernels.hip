@@ -10,17 +10,25 @@
  *   c_{n+1} = G [ M (2 c_n - p_n + V.L c_n) + (I - M) p_n + f_n ],
  *   p_{n+1} = G c_n,   trace row n = R G c_n,   f_n = w_n W V (sources).
  *
- * Its adjoint, backwards in n with t = G alpha_{n+1}:
+ * Its adjoint, backwards in n, with t_n = G alpha_{n+1} and
+ * beta_{n+1} = (I - 2M) t_{n+1} = (I - 2M) G alpha_{n+2}:
  *
- *   alpha_n = G R^T r_n + 2 M t + L (V M t) + G beta_{n+1}
- *   beta_n  = (I - 2M) t
- *   dJ/dV  += M (L c_n) t  (+ w_n W t at the source nodes)
+ *   alpha_n = G R^T r_n + 2 M t_n + L (V M t_n) + G (I - 2M) G alpha_{n+2}
+ *   dJ/dV  += M (L c_n) t_n  (+ w_n W t_n at the source nodes)
  *
- * k_adj2d_step evaluates alpha_n, beta_n and the M (L c_n) t term for every
- * node; k_adj2d_rec adds G R^T r_n from a per-node list of receiver weights
- * (each node's contributions summed by one thread, in receiver order);
- * k_adj2d_src adds the injection term; k_adj2d_chain maps dJ/dV to dJ/ds on
- * the model grid (dV/ds = -V v^2, summed over the padded copies of a node).
+ * so three alpha fields in rotation carry the whole adjoint state.
+ *
+ * k_adj2d_step does one backward step for every node in one launch.  Its
+ * layout follows the forward step kernel: a block is NW waves, lane l of a
+ * wave holds z = z0 + l of RX x rows, z neighbours move across lanes by DPP
+ * shifts with the 2*NOP values beyond the tile edge fetched by the edge
+ * lanes, and x neighbours come from the wave's own rows or NOP halo rows on
+ * each side (served by L2).  Receiver residuals and the injection term are
+ * applied in the same pass from per-tile buckets.  Per node and step it
+ * moves alpha_{n+1}, V, c_n, alpha_{n+2} and dJ/dV in, alpha_n and dJ/dV out.
+ *
+ * k_adj2d_chain maps dJ/dV to dJ/ds on the model grid (dV/ds = -V v^2,
+ * summed over the padded copies of a node).
  */
 #include <hip/hip_runtime.h>
 #include "awefd_hip.h"
@@ -29,126 +37,256 @@
 namespace {
 
 using namespace awefd;
+constexpr int NWA = AWEFD_ADJ_NWAVE;
+constexpr int RXA = AWEFD_ADJ_RX;
 
-/* V_j M_j G_j alpha_{n+1,j}: zero off the interior */
+/* z neighbours at distance 1..NOP of a value held one node per lane; the
+ * edge lanes hold the values beyond the tile in `halo` (lane k < NOP:
+ * z0-1-k, lane 63-k: z0+64+k).  Every lane of the wave must be active. */
 template <int NOP>
-__device__ __forceinline__ float vmt(const awefd_adj2d_args &a, int jz, int jx)
+__device__ __forceinline__ void znb(float c, float halo, float (&zm)[NOP], float (&zp)[NOP])
 {
-    if (jz < NOP || jz >= a.nz - NOP || jx < NOP || jx >= a.nx - NOP) return 0.f;
-    const long j = (long)jx * a.ld + jz;
-    const float gj = a.dabc ? a.sp1[jz] * a.sp2[jx] : 1.f;
-    return a.V[j] * (gj * a.A1[j]);
-}
-
-template <int NOP>
-__global__ __launch_bounds__(256) void k_adj2d_step(const awefd_adj2d_args a)
-{
-    const int iz = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int ix = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int nz = a.nz, nx = a.nx, ld = a.ld;
-    if (iz >= nz || ix >= nx) return;
-    const long i = (long)ix * ld + iz;
-    const bool zin = iz >= NOP && iz < nz - NOP, xin = ix >= NOP && ix < nx - NOP;
-    const bool inter = zin && xin;
-    const float gi = a.dabc ? a.sp1[iz] * a.sp2[ix] : 1.f;
-    const float t = gi * a.A1[i];
-    /* L (V M t) at node i: neighbours j contribute V_j G_j A1_j when interior */
-    float ly = a.c0 * vmt<NOP>(a, iz, ix);
+    float hlo = halo, hhi = halo;
+    zm[0] = shr1(hlo, c);
+    zp[0] = shl1(hhi, c);
 #pragma unroll
-    for (int k = 1; k <= NOP; k++)
-        ly += a.cz[k - 1] * (vmt<NOP>(a, iz + k, ix) + vmt<NOP>(a, iz - k, ix))
-              + a.cx[k - 1] * (vmt<NOP>(a, iz, ix + k) + vmt<NOP>(a, iz, ix - k));
-    a.A0[i] = (inter ? 2.f * t : 0.f) + ly + gi * a.B1[i];
-    a.B0[i] = inter ? -t : t;
-    if (inter) {
-        const float *c = a.Cn;
-        float lc = a.c0 * c[i];
-#pragma unroll
-        for (int k = 1; k <= NOP; k++)
-            lc += a.cz[k - 1] * (c[i + k] + c[i - k]) + a.cx[k - 1] * (c[i + (long)k * ld] + c[i - (long)k * ld]);
-        a.gV[i] += lc * t;
+    for (int k = 1; k < NOP; k++) {
+        hlo = shl1(hlo, hlo);
+        hhi = shr1(hhi, hhi);
+        zm[k] = shr1(hlo, zm[k - 1]);
+        zp[k] = shl1(hhi, zp[k - 1]);
     }
 }
 
-/* alpha_n += G R^T r_n: node k of the receiver list gathers its weights */
-__global__ __launch_bounds__(256) void k_adj2d_rec(const awefd_adj2d_args a)
+/* One backward step n of one grid: alpha_n into A[n % 3], dJ/dV
+ * accumulated. */
+template <int NOP, int RX>
+__device__ __forceinline__ void adj_body(const awefd_adj2d_args &a, int bid, int n)
 {
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= a.nrnode) return;
-    const long i = a.rnode[k];
-    float s = 0.f;
-    for (int e = a.rbeg[k]; e < a.rbeg[k + 1]; e++) s += a.rw[e] * a.res[a.rid[e]];
-    const int iz = (int)(i % a.ld), ix = (int)(i / a.ld);
-    const float gi = a.dabc ? a.sp1[iz] * a.sp2[ix] : 1.f;
-    a.A0[i] += gi * s;
-}
+    float *A0 = a.A[n % 3];
+    const float *A1 = a.A[(n + 1) % 3], *A2 = a.A[(n + 2) % 3];
+    const int k = n / a.K, n0 = k * a.K;
+    const float *Cn = n == n0 ? a.ck + 2L * k * a.nal : a.seg + (long)(n - n0 - 1) * a.nal;
+    const float *res = a.res + (long)n * a.nr;
+    constexpr int H = RX + 2 * NOP;
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    const int lb = xcd_remap(bid, a.nblk);
+    const int tzi = lb % a.ntz, txi = lb / a.ntz;
+    const int nz = a.nz, nx = a.nx;
+    const int z0 = tzi * WAVE, iz = z0 + lane;
+    const int x0 = txi * (NWA * RX) + wv * RX;          /* wave-uniform */
+    int zz = -1;
+    if (lane < NOP) zz = z0 - 1 - lane;
+    else if (lane >= WAVE - NOP) zz = z0 + WAVE + (WAVE - 1 - lane);
+    const bool zv = iz < nz, zhv = zz >= 0 && zz < nz;
+    const unsigned vz = zv ? (unsigned)iz * 4u : OOB;
+    const unsigned vzh = zhv ? (unsigned)zz * 4u : OOB;
+    const bool zi = iz >= NOP && iz < nz - NOP;
+    const bool zhi = zz >= NOP && zz < nz - NOP;
+    const float gz = (a.dabc && zv) ? a.sp1[iz] : 1.f;
+    const float gzh = (a.dabc && zhv) ? a.sp1[zz] : 1.f;
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    const unsigned nbytes = (unsigned)a.ld * (unsigned)nx * 4u;
+    const rsrc_t rA1 = mk_rsrc(A1, nbytes), rA2 = mk_rsrc(A2, nbytes), rV = mk_rsrc(a.V, nbytes);
+    const rsrc_t rC = mk_rsrc(Cn, nbytes), rA0 = mk_rsrc(A0, nbytes), rG = mk_rsrc(a.gV, nbytes);
 
-/* dJ/dV += w_n W t at the injection nodes (t = G alpha_{n+1}) */
-__global__ __launch_bounds__(256) void k_adj2d_src(const awefd_adj2d_args a)
-{
-    const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= a.nsnode) return;
-    const long i = a.snode[k];
-    const int iz = (int)(i % a.ld), ix = (int)(i / a.ld);
-    const float gi = a.dabc ? a.sp1[iz] * a.sp2[ix] : 1.f;
-    float s = 0.f;
-    for (int e = a.sbeg[k]; e < a.sbeg[k + 1]; e++) s += a.wav[(long)a.it * a.wstride + a.sid[e]] * a.sw[e];
-    a.gV[i] += s * (gi * a.A1[i]);
-}
-
-/* dJ/ds on the model grid: sum over the padded nodes copied from model node
- * (kz, kx) of dJ/dV * dV/ds, dV/ds = -V v^2 = -V (V / dt^2) (0 where a free
- * surface fixed V = 0) */
-__global__ __launch_bounds__(256) void k_adj2d_chain(const awefd_adj2d_args a, float *gs)
-{
-    const int kz = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int kx = blockIdx.y * 4 + (threadIdx.x >> 6);
-    if (kz >= a.mz || kx >= a.mx) return;
-    const int nb = a.nb;
-    const int z0 = kz == 0 ? 0 : kz + nb, z1 = kz == a.mz - 1 ? a.nz : kz + nb + 1;
-    const int x0 = kx == 0 ? 0 : kx + nb, x1 = kx == a.mx - 1 ? a.nx : kx + nb + 1;
-    float acc = 0.f;
-    for (int ix = x0; ix < x1; ix++)
-        for (int iz = z0; iz < z1; iz++) {
-            const long i = (long)ix * a.ld + iz;
-            const float v = a.V[i];
-            acc += a.gV[i] * (-v * (v * a.idt2));
+    /* y = V M G alpha_{n+1} and c_n on the RX rows and NOP halo rows each
+     * side; rows outside the grid load zeros (offsets beyond the range) */
+    float y[H], c[H], yh[RX], chl[RX], t[RX], g[RX];
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+        const int xr = x0 - NOP + r;
+        const unsigned so = (unsigned)xr * rowb;
+        const bool xi = xr >= NOP && xr < nx - NOP;
+        const float gx = (a.dabc && xr >= 0 && xr < nx) ? a.sp2[xr] : 1.f;
+        const float a1 = bld(rA1, vz, so), v = bld(rV, vz, so);
+        const float gg = gz * gx;
+        const float tt = gg * a1;
+        y[r] = (zi && xi) ? v * tt : 0.f;
+        c[r] = bld(rC, vz, so);
+        if (r >= NOP && r < NOP + RX) {
+            const int j = r - NOP;
+            t[j] = tt;
+            g[j] = gg;
+            const float a1h = bld(rA1, vzh, so), vh = bld(rV, vzh, so);
+            yh[j] = (zhi && xi) ? vh * ((gzh * gx) * a1h) : 0.f;
+            chl[j] = bld(rC, vzh, so);
         }
-    gs[(long)kx * a.mz + kz] = acc;
+    }
+
+    float al[RX], gv[RX];
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const int r = NOP + j, ix = x0 + j;
+        const bool inter = zi && ix >= NOP && ix < nx - NOP;
+        float zm[NOP], zp[NOP];
+        znb<NOP>(y[r], yh[j], zm, zp);
+        float ly = a.c0 * y[r];
+#pragma unroll
+        for (int k = 1; k <= NOP; k++)
+            ly = ly + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]) + a.cx[k - 1] * (y[r + k] + y[r - k]);
+        znb<NOP>(c[r], chl[j], zm, zp);
+        float lc = a.c0 * c[r];
+#pragma unroll
+        for (int k = 1; k <= NOP; k++)
+            lc = lc + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]) + a.cx[k - 1] * (c[r + k] + c[r - k]);
+        const unsigned so = (unsigned)ix * rowb;
+        const float tb = g[j] * bld(rA2, vz, so);       /* t_{n+1} = G alpha_{n+2} */
+        al[j] = (inter ? 2.f * t[j] : 0.f) + ly + g[j] * (inter ? -tb : tb);
+        gv[j] = inter ? lc * t[j] : 0.f;
+    }
+
+    /* receivers (G R^T r_n) and the injection term, from this tile column's
+     * buckets, in point order */
+    const int col = tzi + a.ntz * txi;
+    for (int e = a.rbeg[col]; e < a.rbeg[col + 1]; e++) {
+        const awefd_inj en = a.rinj[e];
+        const float amp = res[en.src] * en.w;
+#pragma unroll
+        for (int j = 0; j < RX; j++) al[j] = (en.z == iz && en.x == x0 + j) ? al[j] + g[j] * amp : al[j];
+    }
+    for (int e = a.sbeg[col]; e < a.sbeg[col + 1]; e++) {
+        const awefd_inj en = a.sinj[e];
+        const float amp = a.wav[(long)n * a.wstride + en.src] * en.w;
+#pragma unroll
+        for (int j = 0; j < RX; j++) gv[j] = (en.z == iz && en.x == x0 + j) ? gv[j] + amp * t[j] : gv[j];
+    }
+
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const unsigned so = (unsigned)(x0 + j) * rowb;   /* rows >= nx: beyond the range */
+        bst(rA0, vz, so, al[j]);
+        const float g0 = bld(rG, vz, so);
+        bst(rG, vz, so, g0 + gv[j]);
+    }
 }
 
 template <int NOP>
-int launch_step(const awefd_adj2d_args *a, hipStream_t s)
+__global__ __launch_bounds__(NWA * WAVE) void k_adj2d_step(const awefd_adj2d_args a, int n)
 {
-    const dim3 g((a->nz + 63) / 64, (a->nx + 3) / 4);
-    hipLaunchKernelGGL(k_adj2d_step<NOP>, g, dim3(256), 0, s, *a);
+    adj_body<NOP, RXA>(a, blockIdx.x, n);
+}
+
+/* Every shot of a batch in one launch: block b belongs to shot bshot[b] and
+ * is that shot's block b - blk0. */
+template <int NOP>
+__global__ __launch_bounds__(NWA * WAVE) void k_adj2d_batch(const awefd_adj2d_args *__restrict__ shots,
+                                                            const int *__restrict__ bshot, int n)
+{
+    const int s = __builtin_amdgcn_readfirstlane(bshot[blockIdx.x]);
+    const awefd_adj2d_args &a = shots[s];
+    adj_body<NOP, RXA>(a, (int)blockIdx.x - a.blk0, n);
+}
+
+/* dJ/ds on the model grid, summed over the shots in shot order: for each
+ * shot, the sum over the padded nodes copied from model node (kz, kx) of
+ * dJ/dV * dV/ds, dV/ds = -V v^2 = -V (V / dt^2) (0 where a free surface
+ * fixed V = 0).  All shots share the grid. */
+__global__ __launch_bounds__(256) void k_adj2d_chain(const awefd_adj2d_args *__restrict__ shots, int nshot,
+                                                     float *gs, int mz, int mx, int nz, int nx, int ld, int nb)
+{
+    const int kz = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int kx = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (kz >= mz || kx >= mx) return;
+    const int z0 = kz == 0 ? 0 : kz + nb, z1 = kz == mz - 1 ? nz : kz + nb + 1;
+    const int x0 = kx == 0 ? 0 : kx + nb, x1 = kx == mx - 1 ? nx : kx + nb + 1;
+    float tot = 0.f;
+    for (int s = 0; s < nshot; s++) {
+        const awefd_adj2d_args &a = shots[s];
+        float acc = 0.f;
+        for (int ix = x0; ix < x1; ix++)
+            for (int iz = z0; iz < z1; iz++) {
+                const long i = (long)ix * ld + iz;
+                const float v = a.V[i];
+                acc += a.gV[i] * (-v * (v * a.idt2));
+            }
+        tot = s == 0 ? acc : tot + acc;
+    }
+    gs[(long)kx * mz + kz] = tot;
+}
+
+/* Table of copies (or zero fills): block b copies chunk b - cbeg[j] of job j,
+ * 1024 floats, 4 per thread. */
+__global__ __launch_bounds__(256) void k_copy_jobs(const awefd_copy_job *__restrict__ jobs,
+                                                   const long *__restrict__ cbeg, int njob)
+{
+    const long b = blockIdx.x;
+    int lo = 0, hi = njob - 1;
+    while (lo < hi) {                    /* last job with cbeg[j] <= b */
+        const int mid = (lo + hi + 1) >> 1;
+        if (cbeg[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const awefd_copy_job jb = jobs[lo];
+    const long base = (b - cbeg[lo]) * 1024;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const long i = base + q * 256 + threadIdx.x;
+        if (i < jb.n) jb.dst[i] = jb.src ? jb.src[i] : 0.f;
+    }
+}
+
+template <int NOP>
+int launch_step(const awefd_adj2d_args *a, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj2d_step<NOP>, dim3(a->nblk), dim3(NWA * WAVE), 0, s, *a, n);
+    return (int)hipGetLastError();
+}
+
+template <int NOP>
+int launch_batch(const awefd_adj2d_args *shots, const int *bshot, int nblk, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj2d_batch<NOP>, dim3(nblk), dim3(NWA * WAVE), 0, s, shots, bshot, n);
     return (int)hipGetLastError();
 }
 
 } // namespace
 
-extern "C" int awefd_launch_adj2d_step(const awefd_adj2d_args *a, int nop, void *stream)
+extern "C" int awefd_launch_adj2d_step(const awefd_adj2d_args *a, int n, int nop, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    int e;
+    if (a->nblk != a->ntz * a->ntx || a->ntz * 64 < a->nz || a->ntx * AWEFD_ADJ_TX < a->nx || a->K < 1)
+        return (int)hipErrorInvalidValue;
     switch (nop) {
-    case 1: e = launch_step<1>(a, s); break;
-    case 2: e = launch_step<2>(a, s); break;
-    case 3: e = launch_step<3>(a, s); break;
-    case 4: e = launch_step<4>(a, s); break;
-    case 5: e = launch_step<5>(a, s); break;
-    case 6: e = launch_step<6>(a, s); break;
+    case 1: return launch_step<1>(a, n, s);
+    case 2: return launch_step<2>(a, n, s);
+    case 3: return launch_step<3>(a, n, s);
+    case 4: return launch_step<4>(a, n, s);
+    case 5: return launch_step<5>(a, n, s);
+    case 6: return launch_step<6>(a, n, s);
     default: return (int)hipErrorInvalidValue;
     }
-    if (e) return e;
-    if (a->nrnode > 0) hipLaunchKernelGGL(k_adj2d_rec, dim3((a->nrnode + 255) / 256), dim3(256), 0, s, *a);
-    if (a->nsnode > 0) hipLaunchKernelGGL(k_adj2d_src, dim3((a->nsnode + 255) / 256), dim3(256), 0, s, *a);
+}
+
+extern "C" int awefd_launch_adj2d_batch(const awefd_adj2d_args *shots, const int *bshot, int nblk, int n, int nop,
+                                        void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    switch (nop) {
+    case 1: return launch_batch<1>(shots, bshot, nblk, n, s);
+    case 2: return launch_batch<2>(shots, bshot, nblk, n, s);
+    case 3: return launch_batch<3>(shots, bshot, nblk, n, s);
+    case 4: return launch_batch<4>(shots, bshot, nblk, n, s);
+    case 5: return launch_batch<5>(shots, bshot, nblk, n, s);
+    case 6: return launch_batch<6>(shots, bshot, nblk, n, s);
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
+extern "C" int awefd_launch_adj2d_chain(const awefd_adj2d_args *h, const awefd_adj2d_args *shots, int nshot,
+                                        float *gout, void *stream)
+{
+    const dim3 g((h->mz + 63) / 64, (h->mx + 3) / 4);
+    hipLaunchKernelGGL(k_adj2d_chain, g, dim3(256), 0, (hipStream_t)stream, shots, nshot, gout, h->mz, h->mx,
+                       h->nz, h->nx, h->ld, h->nb);
     return (int)hipGetLastError();
 }
 
-extern "C" int awefd_launch_adj2d_chain(const awefd_adj2d_args *a, float *gs, void *stream)
+extern "C" int awefd_launch_copy_jobs(const awefd_copy_job *jobs, const long *cbeg, int njob, long nchunk,
+                                      void *stream)
 {
-    const dim3 g((a->mz + 63) / 64, (a->mx + 3) / 4);
-    hipLaunchKernelGGL(k_adj2d_chain, g, dim3(256), 0, (hipStream_t)stream, *a, gs);
+    if (njob < 1 || nchunk < 1 || nchunk > 0x7fffffffL) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_copy_jobs, dim3((unsigned)nchunk), dim3(256), 0, (hipStream_t)stream, jobs, cbeg, njob);
     return (int)hipGetLastError();
 }

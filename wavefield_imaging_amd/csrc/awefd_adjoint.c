@@ -9,8 +9,9 @@
  * checkpoint memory.  awefd_adjoint_grad visits the segments last to first:
  * it recomputes the fields of a segment from its checkpoint into K-1 segment
  * buffers (out-of-place steps, so the checkpoint survives and no copies are
- * made) and steps the adjoint backwards through them.  Fields held:
- * 2 ceil(nt/K) checkpoints + K-1 segment buffers + 5 adjoint fields.
+ * made) and steps the adjoint backwards through them (three adjoint fields
+ * in rotation and the dJ/dV accumulator).  Fields held: 2 ceil(nt/K)
+ * checkpoints + K-1 segment buffers + 4.
  */
 #define __HIP_PLATFORM_AMD__ 1
 #include <hip/hip_runtime_api.h>
@@ -19,6 +20,7 @@
 #include <string.h>
 
 #include "awefd_model.h"
+#include "awefd_setup.h"
 
 #define HIPCHK(call)                                                                        \
     do {                                                                                    \
@@ -36,23 +38,20 @@ typedef struct {
     int valid;
     float *ck;                /* nck pairs (c_{kK}, c_{kK-1})             */
     float *seg;               /* K-1 fields c_{n0+1} .. c_{n0+K-1}        */
-    float *A[2], *B[2], *gV;  /* adjoint fields and dJ/dV (padded)        */
-    /* per-node receiver / source weight lists */
-    long *rnode, *snode;
-    int *rbeg, *rid, *sbeg, *sid;
-    float *rw, *sw;
-    int nrn, nsn;
+    float *A[3], *gV;         /* alpha fields in rotation, dJ/dV (padded) */
+    /* receiver and injection nodes bucketed per adjoint tile column */
+    awefd_inj *rinj, *sinj;
+    int *rbeg, *sbeg;
     unsigned lists_gen;
     int lists_ok;
+    awefd_adj2d_args *d_args; /* device copy of the kernel arguments    */
 } adj_state;
 
 static void free_lists(adj_state *s)
 {
-    hipFree(s->rnode); hipFree(s->rbeg); hipFree(s->rid); hipFree(s->rw);
-    hipFree(s->snode); hipFree(s->sbeg); hipFree(s->sid); hipFree(s->sw);
-    s->rnode = s->snode = NULL;
-    s->rbeg = s->rid = s->sbeg = s->sid = NULL;
-    s->rw = s->sw = NULL;
+    hipFree(s->rinj); hipFree(s->sinj); hipFree(s->rbeg); hipFree(s->sbeg);
+    s->rinj = s->sinj = NULL;
+    s->rbeg = s->sbeg = NULL;
     s->lists_ok = 0;
 }
 
@@ -62,7 +61,7 @@ void awefd__adjoint_free(awefd_model *m)
     if (!s) return;
     hipSetDevice(m->dev);
     hipFree(s->ck); hipFree(s->seg);
-    hipFree(s->A[0]); hipFree(s->A[1]); hipFree(s->B[0]); hipFree(s->B[1]); hipFree(s->gV);
+    hipFree(s->A[0]); hipFree(s->A[1]); hipFree(s->A[2]); hipFree(s->gV); hipFree(s->d_args);
     free_lists(s);
     free(s);
     m->adj = NULL;
@@ -73,63 +72,38 @@ void awefd_adjoint_release(awefd_model *m)
     awefd__adjoint_free(m);
 }
 
-typedef struct {
-    long node;
-    int order, id;
-    float w;
-} pt_entry;
-
-static int pt_cmp(const void *pa, const void *pb)
+/* Interpolation points of a 2-D grid as (node, id, weight) entries bucketed
+ * per column of 64 x AWEFD_ADJ_TX tiles, uploaded to the device.  id is the
+ * point index (receivers) or the wavelet column (sources: 0 with expl). */
+static int point_buckets(const awefd_model *m, int n, const int *j, const float *w, int same_id, int ntz,
+                         int ntx, awefd_inj **d_inj, int **d_beg)
 {
-    const pt_entry *a = pa, *b = pb;
-    if (a->node != b->node) return a->node < b->node ? -1 : 1;
-    return a->order < b->order ? -1 : (a->order > b->order);
-}
-
-/* Lists of (id, weight) per grid node of n interpolation points: nodes in
- * ascending order, each node's entries in point order. */
-static int node_lists(int n, const int *j, const float *w, int ld, int same_id, long **d_node, int **d_beg,
-                      int **d_id, float **d_w, int *nnode)
-{
-    const size_t ne = (size_t)4 * (n > 0 ? n : 1);
-    pt_entry *e = malloc(sizeof(*e) * ne);
-    long *node = malloc(sizeof(long) * ne);
-    int *beg = malloc(sizeof(int) * (ne + 1)), *id = malloc(sizeof(int) * ne);
-    float *wt = malloc(sizeof(float) * ne);
-    int rc = (!e || !node || !beg || !id || !wt) ? awefd__fail(AWEFD_ENOMEM, "gradient point lists") : AWEFD_OK;
-    int nn = 0;
+    const long ne = 4L * n;
+    const int nbucket = ntz * ntx;
+    awefd_inj_key *keys = malloc(sizeof(*keys) * (ne > 0 ? ne : 1));
+    awefd_inj *e = malloc(sizeof(*e) * (ne > 0 ? ne : 1));
+    int *beg = malloc(sizeof(int) * (nbucket + 1));
+    int rc = (!keys || !e || !beg) ? awefd__fail(AWEFD_ENOMEM, "gradient point buckets") : AWEFD_OK;
     if (!rc) {
         for (int a = 0; a < n; a++)
             for (int q = 0; q < 4; q++) {
-                pt_entry *x = &e[4 * a + q];
-                x->node = (long)(j[3 * a + 1] + ((q >> 1) & 1)) * ld + j[3 * a] + (q & 1);
-                x->order = 4 * a + q;
-                x->id = same_id ? 0 : a;
-                x->w = w[8 * a + q];
+                awefd_inj_key *k = &keys[4L * a + q];
+                k->e.z = j[3 * a] + (q & 1);
+                k->e.x = j[3 * a + 1] + ((q >> 1) & 1);
+                k->e.y = 0;
+                k->e.src = same_id ? 0 : a;
+                k->e.w = w[8 * a + q];
+                k->bucket = k->e.z / 64 + ntz * (k->e.x / AWEFD_ADJ_TX);
             }
-        qsort(e, (size_t)4 * n, sizeof *e, pt_cmp);
-        for (long i = 0; i < 4L * n; i++) {
-            if (i == 0 || e[i].node != e[i - 1].node) {
-                node[nn] = e[i].node;
-                beg[nn] = (int)i;
-                nn++;
-            }
-            id[i] = e[i].id;
-            wt[i] = e[i].w;
-        }
-        beg[nn] = 4 * n;
-        hipError_t er = hipMalloc((void **)d_node, sizeof(long) * (nn > 0 ? nn : 1));
-        if (er == hipSuccess) er = hipMalloc((void **)d_beg, sizeof(int) * (nn + 1));
-        if (er == hipSuccess) er = hipMalloc((void **)d_id, sizeof(int) * ne);
-        if (er == hipSuccess) er = hipMalloc((void **)d_w, sizeof(float) * ne);
-        if (er == hipSuccess && nn) er = hipMemcpy(*d_node, node, sizeof(long) * nn, hipMemcpyHostToDevice);
-        if (er == hipSuccess) er = hipMemcpy(*d_beg, beg, sizeof(int) * (nn + 1), hipMemcpyHostToDevice);
-        if (er == hipSuccess && n) er = hipMemcpy(*d_id, id, sizeof(int) * 4 * n, hipMemcpyHostToDevice);
-        if (er == hipSuccess && n) er = hipMemcpy(*d_w, wt, sizeof(float) * 4 * n, hipMemcpyHostToDevice);
-        if (er != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "gradient point lists: %s", hipGetErrorString(er));
+        awefd_sort_inj(keys, ne, nbucket, e, beg);
+        hipError_t er = hipMalloc((void **)d_inj, sizeof(*e) * (ne > 0 ? ne : 1));
+        if (er == hipSuccess) er = hipMalloc((void **)d_beg, sizeof(int) * (nbucket + 1));
+        if (er == hipSuccess && ne) er = hipMemcpy(*d_inj, e, sizeof(*e) * ne, hipMemcpyHostToDevice);
+        if (er == hipSuccess) er = hipMemcpy(*d_beg, beg, sizeof(int) * (nbucket + 1), hipMemcpyHostToDevice);
+        if (er != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "gradient point buckets: %s", hipGetErrorString(er));
     }
-    *nnode = nn;
-    free(e); free(node); free(beg); free(id); free(wt);
+    (void)m;
+    free(keys); free(e); free(beg);
     return rc;
 }
 
@@ -158,17 +132,22 @@ static int dev_alloc(float **p, size_t bytes)
     return AWEFD_OK;
 }
 
-int awefd_run_ckpt(awefd_model *m, float *d_traces, int ckpt)
+static adj_state *state(const awefd_model *m)
 {
-    int rc = check_model(m, "awefd_run_ckpt");
+    return (adj_state *)m->adj;
+}
+
+int awefd__adj_ckpt_setup(awefd_model *m, int ckpt)
+{
+    int rc = check_model(m, "checkpointed run");
     if (rc) return rc;
     const int nt = m->p.nt;
     const int K = ckpt > 0 ? (ckpt < nt ? ckpt : nt) : (int)ceil(sqrt((double)nt));
     const int nck = (nt + K - 1) / K;
-    const size_t fb = sizeof(float) * (size_t)m->nalloc;
-    awefd_cart_args a;
-    HIPCHK(hipSetDevice(m->dev));
-    adj_state *s = m->adj;
+    if ((size_t)m->nalloc * 4 >= 0x80000000u)
+        return awefd__fail(AWEFD_EINVAL, "gradient: field of %ld nodes exceeds 2 GiB", m->nalloc);
+    if (hipSetDevice(m->dev) != hipSuccess) return awefd__fail(AWEFD_EHIP, "hipSetDevice");
+    adj_state *s = state(m);
     if (!s) {
         s = calloc(1, sizeof *s);
         if (!s) return awefd__fail(AWEFD_ENOMEM, "gradient state");
@@ -182,13 +161,51 @@ int awefd_run_ckpt(awefd_model *m, float *d_traces, int ckpt)
         s->K = K;
         s->nck = nck;
     }
-    if ((rc = dev_alloc(&s->ck, fb * 2 * nck))) goto done;
+    return dev_alloc(&s->ck, sizeof(float) * (size_t)m->nalloc * 2 * nck);
+}
+
+float *awefd__adj_ckpt(const awefd_model *m, int k)
+{
+    return state(m)->ck + (size_t)2 * k * m->nalloc;
+}
+
+int awefd__adj_K(const awefd_model *m)
+{
+    return state(m) ? state(m)->K : 0;
+}
+
+void awefd__adj_mark(awefd_model *m)
+{
+    adj_state *s = state(m);
+    s->nt = m->p.nt;
+    s->gen = m->gen;
+    s->valid = 1;
+}
+
+int awefd__adj_ready(const awefd_model *m, const char *who)
+{
+    int rc = check_model((awefd_model *)m, who);
+    if (rc) return rc;
+    const adj_state *s = state(m);
+    if (!s || !s->valid || s->gen != m->gen || s->nt != m->p.nt)
+        return awefd__fail(AWEFD_ESTATE, "%s: no checkpoints of the current model (run with checkpoints first)", who);
+    return AWEFD_OK;
+}
+
+int awefd_run_ckpt(awefd_model *m, float *d_traces, int ckpt)
+{
+    int rc = awefd__adj_ckpt_setup(m, ckpt);
+    if (rc) return rc;
+    const adj_state *s = state(m);
+    const int nt = m->p.nt, K = s->K;
+    const size_t fb = sizeof(float) * (size_t)m->nalloc;
+    awefd_cart_args a;
     if ((rc = awefd__ensure_traces(m))) goto done;
     if ((rc = awefd_reset(m))) goto done;
     awefd__cart_args(m, &a);
     for (int it = 0; it < nt; it++) {
         if (it % K == 0) {
-            float *c = s->ck + (size_t)2 * (it / K) * m->nalloc;
+            float *c = awefd__adj_ckpt(m, it / K);
             HIPCHK(hipMemcpyAsync(c, m->dC, fb, hipMemcpyDeviceToDevice, m->stream));
             HIPCHK(hipMemcpyAsync(c + m->nalloc, m->dP, fb, hipMemcpyDeviceToDevice, m->stream));
         }
@@ -207,91 +224,130 @@ int awefd_run_ckpt(awefd_model *m, float *d_traces, int ckpt)
         m->steps_done = it + 1;
     }
     HIPCHK(hipStreamSynchronize(m->stream));
-    s->nt = nt;
-    s->gen = m->gen;
-    s->valid = 1;
+    awefd__adj_mark(m);
 done:
     return rc;
 }
 
-int awefd_adjoint_grad(awefd_model *m, const float *d_res, float *d_grad)
+/* Buffers and point buckets of the backward pass (kept across calls). */
+int awefd__adj_prepare(awefd_model *m)
 {
-    int rc = check_model(m, "awefd_adjoint_grad");
-    if (rc) return rc;
-    if (!d_res || !d_grad) return awefd__fail(AWEFD_EINVAL, "awefd_adjoint_grad: NULL buffer");
-    adj_state *s = m->adj;
-    if (!s || !s->valid || s->gen != m->gen || s->nt != m->p.nt)
-        return awefd__fail(AWEFD_ESTATE, "awefd_adjoint_grad: no checkpoints of the current model "
-                                         "(call awefd_run_ckpt first)");
-    const int nt = m->p.nt, K = s->K, nck = s->nck;
+    adj_state *s = state(m);
     const size_t fb = sizeof(float) * (size_t)m->nalloc;
-    awefd_cart_args a;
-    awefd_adj2d_args g;
-    HIPCHK(hipSetDevice(m->dev));
-    if (K > 1 && (rc = dev_alloc(&s->seg, fb * (K - 1)))) goto done;
-    for (int i = 0; i < 2 && !rc; i++) {
-        rc = dev_alloc(&s->A[i], fb);
-        if (!rc) rc = dev_alloc(&s->B[i], fb);
-    }
+    int rc = AWEFD_OK;
+    if (hipSetDevice(m->dev) != hipSuccess) return awefd__fail(AWEFD_EHIP, "hipSetDevice");
+    if (s->K > 1) rc = dev_alloc(&s->seg, fb * (s->K - 1));
+    for (int i = 0; i < 3 && !rc; i++) rc = dev_alloc(&s->A[i], fb);
     if (!rc) rc = dev_alloc(&s->gV, fb);
-    if (rc) goto done;
+    if (!rc && !s->d_args) {
+        hipError_t e = hipMalloc((void **)&s->d_args, sizeof(awefd_adj2d_args));
+        if (e != hipSuccess) rc = awefd__fail(AWEFD_ENOMEM, "gradient arguments: %s", hipGetErrorString(e));
+    }
+    if (rc) return rc;
     if (!s->lists_ok || s->lists_gen != m->gen) {
         free_lists(s);
-        if ((rc = node_lists(m->nr, m->h_rj, m->h_rw, m->ld, 0, &s->rnode, &s->rbeg, &s->rid, &s->rw, &s->nrn)))
-            goto done;
-        if ((rc = node_lists(m->ns, m->h_sj, m->h_sw, m->ld, m->p.expl, &s->snode, &s->sbeg, &s->sid, &s->sw,
-                             &s->nsn)))
-            goto done;
+        const int ntz = (m->np[0] + 63) / 64, ntx = (m->np[1] + AWEFD_ADJ_TX - 1) / AWEFD_ADJ_TX;
+        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntx, &s->rinj, &s->rbeg))) return rc;
+        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, m->p.expl, ntz, ntx, &s->sinj, &s->sbeg))) return rc;
         s->lists_ok = 1;
         s->lists_gen = m->gen;
     }
+    return AWEFD_OK;
+}
 
-    memset(&g, 0, sizeof g);
-    g.V = m->dvel;
-    g.nz = m->np[0];
-    g.nx = m->np[1];
-    g.ld = m->ld;
-    g.mz = m->p.n[0];
-    g.mx = m->p.n[1];
-    g.nb = m->nb;
-    g.dabc = m->p.dabc;
-    g.idt2 = (float)(1.0 / ((double)m->p.dt * m->p.dt));
-    g.sp1 = m->dsp[0];
-    g.sp2 = m->dsp[1];
-    g.c0 = m->c2[0];
+/* Kernel arguments of the backward pass (after awefd__adj_prepare). */
+void awefd__adj_args(const awefd_model *m, const float *d_res, awefd_adj2d_args *g)
+{
+    const adj_state *s = state(m);
+    memset(g, 0, sizeof *g);
+    for (int i = 0; i < 3; i++) g->A[i] = s->A[i];
+    g->ck = s->ck;
+    g->seg = s->seg;
+    g->nal = m->nalloc;
+    g->K = s->K;
+    g->res = d_res;
+    g->nr = m->nr;
+    g->V = m->dvel;
+    g->gV = s->gV;
+    g->nz = m->np[0];
+    g->nx = m->np[1];
+    g->ld = m->ld;
+    g->mz = m->p.n[0];
+    g->mx = m->p.n[1];
+    g->nb = m->nb;
+    g->dabc = m->p.dabc;
+    g->idt2 = (float)(1.0 / ((double)m->p.dt * m->p.dt));
+    g->sp1 = m->dsp[0];
+    g->sp2 = m->dsp[1];
+    g->c0 = m->c2[0];
     for (int k = 1; k <= m->nop; k++) {
-        g.cz[k - 1] = m->c2[k];
-        g.cx[k - 1] = m->c2[m->nop + k];
+        g->cz[k - 1] = m->c2[k];
+        g->cx[k - 1] = m->c2[m->nop + k];
     }
-    g.nrnode = s->nrn;
-    g.rnode = s->rnode;
-    g.rbeg = s->rbeg;
-    g.rid = s->rid;
-    g.rw = s->rw;
-    g.nsnode = s->nsn;
-    g.snode = s->snode;
-    g.sbeg = s->sbeg;
-    g.sid = s->sid;
-    g.sw = s->sw;
-    g.wav = m->d_wav;
-    g.wstride = m->p.expl ? 1 : m->ns;
-    g.gV = s->gV;
+    g->ntz = (m->np[0] + 63) / 64;
+    g->ntx = (m->np[1] + AWEFD_ADJ_TX - 1) / AWEFD_ADJ_TX;
+    g->nblk = g->ntz * g->ntx;
+    g->rbeg = s->rbeg;
+    g->rinj = s->rinj;
+    g->sbeg = s->sbeg;
+    g->sinj = s->sinj;
+    g->wav = m->d_wav;
+    g->wstride = m->p.expl ? 1 : m->ns;
+}
+
+/* Checkpoint segment bases of the batched recompute. */
+void awefd__adj_cart_ckpt(const awefd_model *m, awefd_cart_args *a)
+{
+    const adj_state *s = state(m);
+    a->ck = s->ck;
+    a->seg = s->seg;
+    a->nal = m->nalloc;
+    a->K = s->K;
+}
+
+/* The buffers a backward pass starts from zero: alpha_nt, alpha_{nt+1}
+ * and dJ/dV (src NULL: fill). */
+void awefd__adj_zero_jobs(const awefd_model *m, awefd_copy_job *jobs)
+{
+    const adj_state *s = state(m);
+    const int nt = m->p.nt;
+    float *z[3] = {s->A[nt % 3], s->A[(nt + 1) % 3], s->gV};
+    for (int i = 0; i < 3; i++) {
+        jobs[i].src = NULL;
+        jobs[i].dst = z[i];
+        jobs[i].n = m->nalloc;
+    }
+}
+
+int awefd_adjoint_grad(awefd_model *m, const float *d_res, float *d_grad)
+{
+    int rc = awefd__adj_ready(m, "awefd_adjoint_grad");
+    if (rc) return rc;
+    if (!d_res || !d_grad) return awefd__fail(AWEFD_EINVAL, "awefd_adjoint_grad: NULL buffer");
+    adj_state *s = state(m);
+    const int nt = m->p.nt, K = s->K, nck = s->nck;
+    awefd_cart_args a;
+    awefd_adj2d_args g;
+    if ((rc = awefd__adj_prepare(m))) goto done;
+    awefd__adj_args(m, d_res, &g);
+    HIPCHK(hipMemcpy(s->d_args, &g, sizeof g, hipMemcpyHostToDevice));
     {
-        int e = awefd_launch_fill(s->A[1], m->nalloc, 0.f, m->stream);
-        if (!e) e = awefd_launch_fill(s->B[1], m->nalloc, 0.f, m->stream);
-        if (!e) e = awefd_launch_fill(s->gV, m->nalloc, 0.f, m->stream);
-        if (e) {
-            rc = awefd__fail(AWEFD_EHIP, "gradient reset: %s", hipGetErrorString((hipError_t)e));
-            goto done;
+        awefd_copy_job z[3];
+        awefd__adj_zero_jobs(m, z);
+        for (int i = 0; i < 3; i++) {
+            int e = awefd_launch_fill(z[i].dst, z[i].n, 0.f, m->stream);
+            if (e) {
+                rc = awefd__fail(AWEFD_EHIP, "gradient reset: %s", hipGetErrorString((hipError_t)e));
+                goto done;
+            }
         }
     }
     /* recompute steps: out of place, no trace extraction */
     awefd__cart_args(m, &a);
     a.nr = 0;
-    int cur = 1;   /* A[cur], B[cur] hold alpha_{n+1}, beta_{n+1} */
     for (int k = nck - 1; k >= 0; k--) {
         const int n0 = k * K, n1 = n0 + K < nt ? n0 + K : nt;
-        float *ckc = s->ck + (size_t)2 * k * m->nalloc, *ckp = ckc + m->nalloc;
+        float *ckc = awefd__adj_ckpt(m, k), *ckp = ckc + m->nalloc;
 #define FIELD(n) ((n) == n0 ? ckc : s->seg + (size_t)((n) - n0 - 1) * m->nalloc)
         for (int n = n0; n + 1 < n1; n++) {
             a.C = FIELD(n);
@@ -304,25 +360,17 @@ int awefd_adjoint_grad(awefd_model *m, const float *d_res, float *d_grad)
                 goto done;
             }
         }
+#undef FIELD
         for (int n = n1 - 1; n >= n0; n--) {
-            g.A1 = s->A[cur];
-            g.B1 = s->B[cur];
-            g.A0 = s->A[cur ^ 1];
-            g.B0 = s->B[cur ^ 1];
-            g.Cn = FIELD(n);
-            g.res = d_res + (long)n * m->nr;
-            g.it = n;
-            int e = awefd_launch_adj2d_step(&g, m->nop, m->stream);
+            int e = awefd_launch_adj2d_step(&g, n, m->nop, m->stream);
             if (e) {
                 rc = awefd__fail(AWEFD_EHIP, "adjoint step %d: %s", n, hipGetErrorString((hipError_t)e));
                 goto done;
             }
-            cur ^= 1;
         }
-#undef FIELD
     }
     {
-        int e = awefd_launch_adj2d_chain(&g, d_grad, m->stream);
+        int e = awefd_launch_adj2d_chain(&g, s->d_args, 1, d_grad, m->stream);
         if (e) {
             rc = awefd__fail(AWEFD_EHIP, "gradient chain: %s", hipGetErrorString((hipError_t)e));
             goto done;

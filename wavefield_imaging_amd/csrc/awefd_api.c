@@ -777,6 +777,16 @@ int awefd_run(awefd_model *m, float *traces, int jsnap, awefd_snap_fn cb, void *
     return AWEFD_OK;
 }
 
+int awefd_traces_dev(awefd_model *m, float **d_traces)
+{
+    if (!m || !d_traces) return awefd__fail(AWEFD_EINVAL, "awefd_traces_dev: NULL argument");
+    HIPCHK(hipSetDevice(m->dev));
+    int rc = awefd__ensure_traces(m);
+    if (rc) return rc;
+    *d_traces = m->d_traces;
+    return AWEFD_OK;
+}
+
 int awefd_get_traces(awefd_model *m, float *host)
 {
     if (!m || !host) return awefd__fail(AWEFD_EINVAL, "awefd_get_traces: NULL argument");

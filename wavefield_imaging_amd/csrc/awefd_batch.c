@@ -56,6 +56,26 @@ struct awefd_batch {
     hipGraphExec_t gexec;
     int graph_ok;
     double tune_ms[4][NGC];
+    int *grp;                        /* group of each shot                       */
+    awefd_cart_args *h;              /* host copy of d_shots                     */
+    /* gradient: checkpointed forward and backward pass (awefd_adjoint.c holds
+     * each shot's buffers) */
+    int K, nck;
+    awefd_copy_job *d_jobs;          /* checkpoint copies per (group, k)         */
+    long *d_cbeg;
+    long *joff, *jnch;               /* per (group, k): first job, chunks        */
+    awefd_adj2d_args *ha, *d_adj;    /* per-shot adjoint arguments               */
+    int *d_abshot;                   /* adjoint launch: block -> shot            */
+    long *agoff;
+    int *agnblk;
+    awefd_copy_job *d_zjobs;         /* zero fills starting the backward pass    */
+    long *d_zcbeg, znch;
+    long abshot_cap;
+    float **fwd_ck;                  /* checkpoint bases the forward graph holds */
+    hipGraph_t gfwd, gbwd;
+    hipGraphExec_t xfwd, xbwd;
+    int fwd_ok, bwd_ok;
+    float *bwd_gout;
 };
 
 static void dfree(void *pp)
@@ -71,6 +91,24 @@ void awefd_batch_destroy(awefd_batch *b)
     hipSetDevice(b->dev);
     if (b->gexec) hipGraphExecDestroy(b->gexec);
     if (b->graph) hipGraphDestroy(b->graph);
+    if (b->xfwd) hipGraphExecDestroy(b->xfwd);
+    if (b->gfwd) hipGraphDestroy(b->gfwd);
+    if (b->xbwd) hipGraphExecDestroy(b->xbwd);
+    if (b->gbwd) hipGraphDestroy(b->gbwd);
+    dfree(&b->d_jobs);
+    dfree(&b->d_cbeg);
+    dfree(&b->d_adj);
+    dfree(&b->d_abshot);
+    dfree(&b->d_zjobs);
+    dfree(&b->d_zcbeg);
+    free(b->joff);
+    free(b->jnch);
+    free(b->ha);
+    free(b->agoff);
+    free(b->agnblk);
+    free(b->grp);
+    free(b->h);
+    free(b->fwd_ck);
     for (int s = 0; s < b->nshot; s++) {
         if (b->d_inj) dfree(&b->d_inj[s]);
         if (b->d_ibeg) dfree(&b->d_ibeg[s]);
@@ -223,22 +261,27 @@ static int batch_build(awefd_batch *b, int rx, int group_mb)
         if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "batch table upload: %s", hipGetErrorString(e));
     }
     free(bshot);
-    free(h);
-    free(grp);
     free(nb);
     if (!rc) {
         free(b->goff);
         free(b->gnblk);
+        free(b->h);
+        free(b->grp);
         b->goff = goff;
         b->gnblk = gnblk;
+        b->h = h;
+        b->grp = grp;
         b->ngroup = ng;
         b->rx = rx;
         b->group_mb = group_mb;
     } else {
         free(goff);
         free(gnblk);
+        free(h);
+        free(grp);
     }
-    b->graph_ok = 0;   /* a captured graph refers to the tables just replaced */
+    /* captured graphs refer to the tables just replaced */
+    b->graph_ok = b->fwd_ok = b->bwd_ok = 0;
     return rc;
 }
 
@@ -360,5 +403,274 @@ int awefd_batch_run(awefd_batch *b)
         m->dC = b->nt % 2 ? b->u0[s] : b->u1[s];
         m->steps_done = b->nt;
     }
+    return AWEFD_OK;
+}
+
+/* ---- Gradient: checkpointed forward and backward pass of every shot ---- */
+
+static long chunks(long n)
+{
+    return (n + 1023) / 1024;
+}
+
+/* Checkpoint copy tables: for group g and checkpoint k (step kK), every shot
+ * of g copies its current and previous field (buffers by step parity, as
+ * k_cart_batch2 alternates them) into its checkpoint pair k. */
+static int ckpt_tables(awefd_batch *b)
+{
+    const int ng = b->ngroup, nck = b->nck;
+    long njob = 2L * b->nshot * nck;
+    awefd_copy_job *jobs = malloc(sizeof(*jobs) * njob);
+    long *cbeg = malloc(sizeof(long) * njob);
+    long *joff = malloc(sizeof(long) * ng * nck), *jnch = malloc(sizeof(long) * ng * nck);
+    if (!jobs || !cbeg || !joff || !jnch) {
+        free(jobs); free(cbeg); free(joff); free(jnch);
+        return awefd__fail(AWEFD_ENOMEM, "checkpoint tables");
+    }
+    long j = 0;
+    for (int g = 0; g < ng; g++)
+        for (int k = 0; k < nck; k++) {
+            const int it = k * b->K;
+            joff[g * nck + k] = j;
+            long c = 0;
+            for (int s = 0; s < b->nshot; s++) {
+                if (b->grp[s] != g) continue;
+                awefd_model *m = b->m[s];
+                float *ck = awefd__adj_ckpt(m, k);
+                const float *cur = it & 1 ? b->u0[s] : b->u1[s];
+                const float *prev = it & 1 ? b->u1[s] : b->u0[s];
+                jobs[j] = (awefd_copy_job){cur, ck, m->nalloc};
+                cbeg[j++] = c;
+                c += chunks(m->nalloc);
+                jobs[j] = (awefd_copy_job){prev, ck + m->nalloc, m->nalloc};
+                cbeg[j++] = c;
+                c += chunks(m->nalloc);
+            }
+            jnch[g * nck + k] = c;
+        }
+    dfree(&b->d_jobs);
+    dfree(&b->d_cbeg);
+    hipError_t e = hipMalloc((void **)&b->d_jobs, sizeof(*jobs) * njob);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_cbeg, sizeof(long) * njob);
+    if (e == hipSuccess) e = hipMemcpy(b->d_jobs, jobs, sizeof(*jobs) * njob, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_cbeg, cbeg, sizeof(long) * njob, hipMemcpyHostToDevice);
+    free(jobs);
+    free(cbeg);
+    free(b->joff);
+    free(b->jnch);
+    b->joff = joff;
+    b->jnch = jnch;
+    if (e != hipSuccess) return awefd__fail(AWEFD_EHIP, "checkpoint tables: %s", hipGetErrorString(e));
+    return AWEFD_OK;
+}
+
+static int capture_fwd(awefd_batch *b)
+{
+    if (b->xfwd && b->fwd_ok) return AWEFD_OK;
+    if (b->xfwd) hipGraphExecDestroy(b->xfwd);
+    if (b->gfwd) hipGraphDestroy(b->gfwd);
+    b->xfwd = NULL;
+    b->gfwd = NULL;
+    int rc = ckpt_tables(b);
+    if (rc) return rc;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = batch_reset(b);
+    for (int g = 0; g < b->ngroup && !rc; g++)
+        for (int it = 0; it < b->nt && !rc; it++) {
+            int e = 0;
+            if (it % b->K == 0) {
+                const long t = (long)g * b->nck + it / b->K;
+                const int nj = (int)((t + 1 < (long)b->ngroup * b->nck ? b->joff[t + 1] : 2L * b->nshot * b->nck)
+                                     - b->joff[t]);
+                if (nj > 0)
+                    e = awefd_launch_copy_jobs(b->d_jobs + b->joff[t], b->d_cbeg + b->joff[t], nj, b->jnch[t],
+                                               b->stream);
+            }
+            if (!e)
+                e = awefd_launch_cart_batch2(b->d_shots, b->d_bshot + b->goff[g], b->gnblk[g], it, b->nop, b->sse,
+                                             b->rx, b->stream);
+            if (e) rc = awefd__fail(AWEFD_EHIP, "checkpointed batch step %d: %s", it, hipGetErrorString((hipError_t)e));
+        }
+    hipGraph_t gr = NULL;
+    hipError_t e = hipStreamEndCapture(b->stream, &gr);
+    if (rc) {
+        if (gr) hipGraphDestroy(gr);
+        return rc;
+    }
+    if (e != hipSuccess) return awefd__fail(AWEFD_EHIP, "checkpointed batch capture: %s", hipGetErrorString(e));
+    b->gfwd = gr;
+    HIPCHK(hipGraphInstantiate(&b->xfwd, gr, NULL, NULL, 0));
+    b->fwd_ok = 1;
+    return AWEFD_OK;
+}
+
+int awefd_batch_run_ckpt(awefd_batch *b, int ckpt)
+{
+    if (!b) return awefd__fail(AWEFD_EINVAL, "NULL batch");
+    HIPCHK(hipSetDevice(b->dev));
+    int rc;
+    for (int s = 0; s < b->nshot; s++)
+        if ((rc = awefd__adj_ckpt_setup(b->m[s], ckpt))) return rc;
+    const int K = awefd__adj_K(b->m[0]);
+    if (!b->rx && (rc = batch_build(b, 8, 0))) return rc;
+    if (K != b->K) {
+        b->K = K;
+        b->nck = (b->nt + K - 1) / K;
+        b->fwd_ok = b->bwd_ok = 0;
+    }
+    /* the graph holds every shot's checkpoint base: recapture if one moved */
+    if (!b->fwd_ck && !(b->fwd_ck = calloc(b->nshot, sizeof(float *))))
+        return awefd__fail(AWEFD_ENOMEM, "batch checkpoint bases");
+    for (int s = 0; s < b->nshot; s++)
+        if (b->fwd_ck[s] != awefd__adj_ckpt(b->m[s], 0)) b->fwd_ok = 0;
+    if ((rc = capture_fwd(b))) return rc;
+    for (int s = 0; s < b->nshot; s++) b->fwd_ck[s] = awefd__adj_ckpt(b->m[s], 0);
+    HIPCHK(hipGraphLaunch(b->xfwd, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int s = 0; s < b->nshot; s++) {
+        awefd_model *m = b->m[s];
+        m->dP = b->nt % 2 ? b->u1[s] : b->u0[s];
+        m->dC = b->nt % 2 ? b->u0[s] : b->u1[s];
+        m->steps_done = b->nt;
+        awefd__adj_mark(m);
+    }
+    return AWEFD_OK;
+}
+
+/* Adjoint tables: arguments per shot and the block -> shot map of each
+ * group's launch. */
+static int adj_tables(awefd_batch *b, const float *const *d_res)
+{
+    const int ns = b->nshot, ng = b->ngroup;
+    awefd_adj2d_args *ha = malloc(sizeof(*ha) * ns);
+    long *agoff = calloc(ng + 1, sizeof(long));
+    int *agnblk = calloc(ng, sizeof(int));
+    int rc = (!ha || !agoff || !agnblk) ? awefd__fail(AWEFD_ENOMEM, "adjoint tables") : AWEFD_OK;
+    for (int s = 0; s < ns && !rc; s++) {
+        awefd__adj_args(b->m[s], d_res[s], &ha[s]);
+        ha[s].blk0 = agnblk[b->grp[s]];
+        if ((long)agnblk[b->grp[s]] + ha[s].nblk > 0x7fffffffL) rc = awefd__fail(AWEFD_EINVAL, "adjoint: too many blocks");
+        else agnblk[b->grp[s]] += ha[s].nblk;
+    }
+    for (int g = 0; g < ng && !rc; g++) agoff[g + 1] = agoff[g] + agnblk[g];
+    int *bshot = rc ? NULL : malloc(sizeof(int) * (agoff[ng] > 0 ? agoff[ng] : 1));
+    if (!rc && !bshot) rc = awefd__fail(AWEFD_ENOMEM, "adjoint block table");
+    for (int s = 0; s < ns && !rc; s++)
+        for (int k = 0; k < ha[s].nblk; k++) bshot[agoff[b->grp[s]] + ha[s].blk0 + k] = s;
+    /* zero fills of every shot */
+    awefd_copy_job *z = rc ? NULL : malloc(sizeof(*z) * 3 * ns);
+    long *zc = rc ? NULL : malloc(sizeof(long) * 3 * ns);
+    if (!rc && (!z || !zc)) rc = awefd__fail(AWEFD_ENOMEM, "adjoint fills");
+    long nch = 0;
+    for (int s = 0; s < ns && !rc; s++) {
+        awefd__adj_zero_jobs(b->m[s], z + 3 * s);
+        for (int i = 0; i < 3; i++) {
+            zc[3 * s + i] = nch;
+            nch += chunks(z[3 * s + i].n);
+        }
+    }
+    if (!rc) {
+        hipError_t e = hipSuccess;
+        /* allocated once per block-table size: a captured graph holds these
+         * addresses */
+        if (b->abshot_cap < agoff[ng]) {
+            dfree(&b->d_abshot);
+            b->abshot_cap = 0;
+            b->bwd_ok = 0;
+            e = hipMalloc((void **)&b->d_abshot, sizeof(int) * (agoff[ng] > 0 ? agoff[ng] : 1));
+            if (e == hipSuccess) b->abshot_cap = agoff[ng];
+        }
+        if (e == hipSuccess && !b->d_adj) {
+            b->bwd_ok = 0;
+            e = hipMalloc((void **)&b->d_adj, sizeof(*ha) * ns);
+            if (e == hipSuccess) e = hipMalloc((void **)&b->d_zjobs, sizeof(*z) * 3 * ns);
+            if (e == hipSuccess) e = hipMalloc((void **)&b->d_zcbeg, sizeof(long) * 3 * ns);
+        }
+        if (e == hipSuccess) e = hipMemcpy(b->d_adj, ha, sizeof(*ha) * ns, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_abshot, bshot, sizeof(int) * agoff[ng], hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_zjobs, z, sizeof(*z) * 3 * ns, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_zcbeg, zc, sizeof(long) * 3 * ns, hipMemcpyHostToDevice);
+        /* the recompute reads each shot's checkpoint and segment bases */
+        for (int s = 0; s < ns; s++) awefd__adj_cart_ckpt(b->m[s], &b->h[s]);
+        if (e == hipSuccess) e = hipMemcpy(b->d_shots, b->h, sizeof(*b->h) * ns, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "adjoint tables: %s", hipGetErrorString(e));
+    }
+    free(bshot);
+    free(z);
+    free(zc);
+    if (!rc) {
+        free(b->ha);
+        free(b->agoff);
+        free(b->agnblk);
+        b->ha = ha;
+        b->agoff = agoff;
+        b->agnblk = agnblk;
+        b->znch = nch;
+    } else {
+        free(ha);
+        free(agoff);
+        free(agnblk);
+    }
+    return rc;
+}
+
+static int capture_bwd(awefd_batch *b, float *d_grad)
+{
+    if (b->xbwd && b->bwd_ok && b->bwd_gout == d_grad) return AWEFD_OK;
+    if (b->xbwd) hipGraphExecDestroy(b->xbwd);
+    if (b->gbwd) hipGraphDestroy(b->gbwd);
+    b->xbwd = NULL;
+    b->gbwd = NULL;
+    int rc = AWEFD_OK;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int e = awefd_launch_copy_jobs(b->d_zjobs, b->d_zcbeg, 3 * b->nshot, b->znch, b->stream);
+    for (int g = 0; g < b->ngroup && !e; g++)
+        for (int k = b->nck - 1; k >= 0 && !e; k--) {
+            const int n0 = k * b->K, n1 = n0 + b->K < b->nt ? n0 + b->K : b->nt;
+            for (int n = n0; n + 1 < n1 && !e; n++)
+                e = awefd_launch_cart_batch_rc(b->d_shots, b->d_bshot + b->goff[g], b->gnblk[g], n, b->nop, b->sse,
+                                               b->rx, b->stream);
+            for (int n = n1 - 1; n >= n0 && !e; n--)
+                e = awefd_launch_adj2d_batch(b->d_adj, b->d_abshot + b->agoff[g], b->agnblk[g], n, b->nop, b->stream);
+        }
+    if (!e) e = awefd_launch_adj2d_chain(&b->ha[0], b->d_adj, b->nshot, d_grad, b->stream);
+    if (e) rc = awefd__fail(AWEFD_EHIP, "batched backward pass: %s", hipGetErrorString((hipError_t)e));
+    hipGraph_t gr = NULL;
+    hipError_t he = hipStreamEndCapture(b->stream, &gr);
+    if (rc) {
+        if (gr) hipGraphDestroy(gr);
+        return rc;
+    }
+    if (he != hipSuccess) return awefd__fail(AWEFD_EHIP, "batched backward capture: %s", hipGetErrorString(he));
+    b->gbwd = gr;
+    HIPCHK(hipGraphInstantiate(&b->xbwd, gr, NULL, NULL, 0));
+    b->bwd_ok = 1;
+    b->bwd_gout = d_grad;
+    return AWEFD_OK;
+}
+
+int awefd_batch_adjoint_grad(awefd_batch *b, const float *const *d_res, float *d_grad)
+{
+    if (!b || !d_res || !d_grad) return awefd__fail(AWEFD_EINVAL, "awefd_batch_adjoint_grad: NULL argument");
+    HIPCHK(hipSetDevice(b->dev));
+    const awefd_model *m0 = b->m[0];
+    int rc;
+    for (int s = 0; s < b->nshot; s++) {
+        awefd_model *m = b->m[s];
+        if (!d_res[s]) return awefd__fail(AWEFD_EINVAL, "awefd_batch_adjoint_grad: residual %d is NULL", s);
+        if ((rc = awefd__adj_ready(m, "awefd_batch_adjoint_grad"))) return rc;
+        if (awefd__adj_K(m) != b->K)
+            return awefd__fail(AWEFD_ESTATE, "awefd_batch_adjoint_grad: shot %d was not run by awefd_batch_run_ckpt", s);
+        if (m->p.n[0] != m0->p.n[0] || m->p.n[1] != m0->p.n[1] || m->nb != m0->nb || m->ld != m0->ld)
+            return awefd__fail(AWEFD_EINVAL, "awefd_batch_adjoint_grad: shot %d has another grid (the gradient "
+                                             "sums over shots on one grid)", s);
+        if ((rc = awefd__adj_prepare(m))) return rc;
+    }
+    /* the tables hold the residual pointers and the buffers just prepared;
+     * the captured graph reads them when it runs */
+    if ((rc = adj_tables(b, d_res))) return rc;
+    if ((rc = capture_bwd(b, d_grad))) return rc;
+    HIPCHK(hipGraphLaunch(b->xbwd, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
     return AWEFD_OK;
 }

@@ -65,37 +65,64 @@ typedef struct {
     const float *rw;         /* 8 weights per receiver (4 used in 2-D)       */
     float *trow;             /* traces + it*nr (batched: the traces base)     */
     int blk0;                /* batched: first block of this shot             */
+    /* batched recompute of a checkpoint segment: c_n of segment k is
+     * ck + 2k nal at n = kK (c_{kK-1} at ck + (2k+1) nal), else
+     * seg + (n - kK - 1) nal */
+    const float *ck;
+    float *seg;
+    long nal;
+    int K;
 } awefd_cart_args;
 
 /* Adjoint of the cartesian 2-D loop (adj_kernels.hip). */
+#define AWEFD_ADJ_NWAVE 4        /* waves per block of the adjoint step kernel   */
+#ifndef AWEFD_ADJ_RX
+#define AWEFD_ADJ_RX 2           /* x rows per wave of the adjoint step kernel   */
+#endif
+#define AWEFD_ADJ_TX (AWEFD_ADJ_NWAVE * AWEFD_ADJ_RX)
 typedef struct {
-    const float *A1, *B1;    /* alpha_{n+1}, beta_{n+1}                       */
-    float *A0, *B0;          /* alpha_n, beta_n                               */
-    const float *Cn;         /* c_n: the current field at the start of step n */
+    /* per-step pointers follow from the step n: alpha_n lives in A[n % 3];
+     * c_n is checkpoint k = n / K (ck + 2k nal) at n = kK, else segment
+     * buffer seg + (n - kK - 1) nal; residual row n at res + n nr */
+    float *A[3];
+    const float *ck, *seg;
+    long nal;                /* floats per field                              */
+    int K;
+    const float *res;        /* residual rows (nt x nr)                       */
+    int nr;
     const float *V;          /* (v dt)^2                                      */
     float *gV;               /* dJ/dV accumulator (padded grid)               */
+    float *gout;             /* dJ/ds output of the chain (model grid)        */
     int nz, nx, ld, mz, mx, nb, dabc;
     float idt2;              /* 1/dt^2                                        */
     const float *sp1, *sp2;  /* sponge factors per axis                       */
     float c0, cz[AWEFD_MAX_NOP], cx[AWEFD_MAX_NOP];
-    /* receivers: per node, (receiver, weight) in receiver order */
-    int nrnode;
-    const long *rnode;
-    const int *rbeg, *rid;
-    const float *rw;
-    const float *res;        /* residual row n (nr floats)                    */
-    /* injection nodes: per node, (wavelet column, weight) */
-    int nsnode;
-    const long *snode;
-    const int *sbeg, *sid;
-    const float *sw;
+    int ntz, ntx, nblk;      /* 64 x AWEFD_ADJ_TX tiles                       */
+    /* receiver and injection nodes, bucketed per tile column like the
+     * forward injection (src = receiver index, or wavelet column) */
+    const int *rbeg, *sbeg;
+    const awefd_inj *rinj, *sinj;
     const float *wav;
     long wstride;
-    int it;
+    int blk0;                /* batched: first block of this shot             */
 } awefd_adj2d_args;
 
-int awefd_launch_adj2d_step(const awefd_adj2d_args *a, int nop, void *stream);
-int awefd_launch_adj2d_chain(const awefd_adj2d_args *a, float *gs, void *stream);
+int awefd_launch_adj2d_step(const awefd_adj2d_args *a, int n, int nop, void *stream);
+/* shots: device array of nshot argument records; bshot: block -> shot */
+int awefd_launch_adj2d_batch(const awefd_adj2d_args *shots, const int *bshot, int nblk, int n, int nop,
+                             void *stream);
+/* gout (model grid) = sum over the shots of dJ/ds; h: host copy of shot 0's
+ * record (grid sizes), shots: device records */
+int awefd_launch_adj2d_chain(const awefd_adj2d_args *h, const awefd_adj2d_args *shots, int nshot, float *gout,
+                             void *stream);
+/* dst_j = src_j (or 0 where src_j is NULL) for every job of a table:
+ * cbeg[j] .. cbeg[j+1] are job j's 1024-float chunks */
+typedef struct {
+    const float *src;
+    float *dst;
+    long n;
+} awefd_copy_job;
+int awefd_launch_copy_jobs(const awefd_copy_job *jobs, const long *cbeg, int njob, long nchunk, void *stream);
 
 /* Polar (2-D) and spherical (3-D) kernels.  Axes: 1 = r (fastest), 2 = theta,
  * 3 = phi.  Nodes within AWEFD_SPHER_SHELL layers of a face form the shell:
@@ -159,6 +186,10 @@ int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void *stream);
 int awefd_launch_fill(float *p, long n, float v, void *stream);
 /* Batched 2-D step over many shots (cart_kernels.hip): shots = device array
  * of per-shot arguments, bshot = shot of every block, rx in {2,4,8,16}. */
+/* recompute step n of every shot inside its checkpoint segment (out of
+ * place, no extraction), same tables as awefd_launch_cart_batch2 */
+int awefd_launch_cart_batch_rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n, int nop,
+                               int sse, int rx, void *stream);
 int awefd_launch_cart_batch2(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int nop,
                              int sse, int rx, void *stream);
 /* resident blocks per CU of the cartesian step kernel (0 if unknown) */

@@ -79,7 +79,17 @@ int  awefd__ensure_traces(awefd_model *m);
 int  awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj **inj, int **ibeg,
                          int *nbucket, long *ninj);
 
-void awefd__adjoint_free(awefd_model *m);
+/* gradient internals shared with the batched path (awefd_adjoint.c) */
+void   awefd__adjoint_free(awefd_model *m);
+int    awefd__adj_ckpt_setup(awefd_model *m, int ckpt);
+float *awefd__adj_ckpt(const awefd_model *m, int k);
+int    awefd__adj_K(const awefd_model *m);
+void   awefd__adj_mark(awefd_model *m);
+int    awefd__adj_ready(const awefd_model *m, const char *who);
+int    awefd__adj_prepare(awefd_model *m);
+void   awefd__adj_args(const awefd_model *m, const float *d_res, awefd_adj2d_args *g);
+void   awefd__adj_cart_ckpt(const awefd_model *m, awefd_cart_args *a);
+void   awefd__adj_zero_jobs(const awefd_model *m, awefd_copy_job *jobs);
 
 /* spherical back end (spher_host.c) */
 int  awefd__spher_create(awefd_model *m);

@@ -442,16 +442,29 @@ __global__ __launch_bounds__(NTHR0) void k_cart_step(const awefd_cart_args a)
  * with the parity of the step, and its trace row is traces + it*nr.  Tile
  * shape (RX rows per wave, NW waves per block) is a template parameter, chosen
  * per batch by the host autotuner. */
-template <int NOP, bool SSE, int RX, int NW>
+template <int NOP, bool SSE, int RX, int NW, bool RC>
 __global__ __launch_bounds__(NW * WAVE) void k_cart_batch2(const awefd_cart_args *__restrict__ shots,
                                                            const int *__restrict__ bshot, int it)
 {
     const int s = __builtin_amdgcn_readfirstlane(bshot[blockIdx.x]);
     const awefd_cart_args &a = shots[s];
-    const bool odd = it & 1;
-    float *P = odd ? (float *)a.C : a.P;
-    const float *C = odd ? a.P : a.C;
-    cart_step_body<NOP, 2, SSE, false, RX, NW>(a, P, C, it, a.trow + (long)it * a.nr, (int)blockIdx.x - a.blk0, P);
+    const int bid = (int)blockIdx.x - a.blk0;
+    if (!RC) {
+        const bool odd = it & 1;
+        float *P = odd ? (float *)a.C : a.P;
+        const float *C = odd ? a.P : a.C;
+        cart_step_body<NOP, 2, SSE, false, RX, NW>(a, P, C, it, a.trow + (long)it * a.nr, bid, P);
+        return;
+    }
+    /* recompute of step it inside its checkpoint segment, out of place;
+     * the fields are those of the forward run, so no traces are written */
+    if (bid >= a.nblk) return;
+    const int k = it / a.K, n0 = k * a.K;
+    const float *ckc = a.ck + 2L * k * a.nal;
+    float *cur = it == n0 ? (float *)ckc : a.seg + (long)(it - n0 - 1) * a.nal;
+    float *prev = it == n0 ? (float *)ckc + a.nal : (it - 1 == n0 ? (float *)ckc : a.seg + (long)(it - n0 - 2) * a.nal);
+    float *next = a.seg + (long)(it - n0) * a.nal;
+    cart_step_body<NOP, 2, SSE, false, RX, NW>(a, prev, cur, it, a.trow, bid, next);
 }
 
 /* z-high one-way ABC as a separate pass, for grids whose last z tile is too
@@ -585,25 +598,40 @@ int launch_abc(const awefd_cart_args *a, hipStream_t s)
     return (int)hipGetLastError();
 }
 
-template <int NOP, bool SSE>
+template <int NOP, bool SSE, bool RC>
 int launch_batch_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int rx, hipStream_t s)
 {
     constexpr int NW = AWEFD_BATCH_NWAVE;
     switch (rx) {
-    case 2: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 2, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
-    case 4: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 4, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
-    case 8: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 8, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
-    case 16: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 16, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    case 2: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 2, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    case 4: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 4, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    case 8: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 8, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
+    case 16: hipLaunchKernelGGL((k_cart_batch2<NOP, SSE, 16, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it); break;
     default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();
 }
 
-template <int NOP>
+template <int NOP, bool RC>
 int launch_batch_nop(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int sse, int rx, hipStream_t s)
 {
-    return sse ? launch_batch_sse<NOP, true>(shots, bshot, nblk, it, rx, s)
-               : launch_batch_sse<NOP, false>(shots, bshot, nblk, it, rx, s);
+    return sse ? launch_batch_sse<NOP, true, RC>(shots, bshot, nblk, it, rx, s)
+               : launch_batch_sse<NOP, false, RC>(shots, bshot, nblk, it, rx, s);
+}
+
+template <bool RC>
+int launch_batch(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int nop, int sse, int rx,
+                 hipStream_t s)
+{
+    switch (nop) {
+    case 1: return launch_batch_nop<1, RC>(shots, bshot, nblk, it, sse, rx, s);
+    case 2: return launch_batch_nop<2, RC>(shots, bshot, nblk, it, sse, rx, s);
+    case 3: return launch_batch_nop<3, RC>(shots, bshot, nblk, it, sse, rx, s);
+    case 4: return launch_batch_nop<4, RC>(shots, bshot, nblk, it, sse, rx, s);
+    case 5: return launch_batch_nop<5, RC>(shots, bshot, nblk, it, sse, rx, s);
+    case 6: return launch_batch_nop<6, RC>(shots, bshot, nblk, it, sse, rx, s);
+    default: return (int)hipErrorInvalidValue;
+    }
 }
 
 } // namespace
@@ -611,16 +639,13 @@ int launch_batch_nop(const awefd_cart_args *shots, const int *bshot, int nblk, i
 extern "C" int awefd_launch_cart_batch2(const awefd_cart_args *shots, const int *bshot, int nblk, int it,
                                         int nop, int sse, int rx, void *stream)
 {
-    hipStream_t s = (hipStream_t)stream;
-    switch (nop) {
-    case 1: return launch_batch_nop<1>(shots, bshot, nblk, it, sse, rx, s);
-    case 2: return launch_batch_nop<2>(shots, bshot, nblk, it, sse, rx, s);
-    case 3: return launch_batch_nop<3>(shots, bshot, nblk, it, sse, rx, s);
-    case 4: return launch_batch_nop<4>(shots, bshot, nblk, it, sse, rx, s);
-    case 5: return launch_batch_nop<5>(shots, bshot, nblk, it, sse, rx, s);
-    case 6: return launch_batch_nop<6>(shots, bshot, nblk, it, sse, rx, s);
-    default: return (int)hipErrorInvalidValue;
-    }
+    return launch_batch<false>(shots, bshot, nblk, it, nop, sse, rx, (hipStream_t)stream);
+}
+
+extern "C" int awefd_launch_cart_batch_rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n,
+                                          int nop, int sse, int rx, void *stream)
+{
+    return launch_batch<true>(shots, bshot, nblk, n, nop, sse, rx, (hipStream_t)stream);
 }
 
 extern "C" int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndim, int sse,

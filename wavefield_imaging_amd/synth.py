@@ -186,3 +186,28 @@ def batch2d(nshot=256, nt=2000, seed=2024, scale=1.0):
                    free=False, cden=True, fdorder=4, expl=False)
         shots.append(dict(kind="cart2d", par=par, vel=vel, sou=sou, rec=rec, seed=mseed))
     return dict(shots=shots, wav=ricker(nt, DT))
+
+
+def adjoint2d(scale=1.0, nt=None, nshot=64, seed=4096):
+    """Gradient workload: one 1000 x 2000 (z, x) grid, nb = 50, nt = 4000,
+    ``nshot`` sources evenly spaced along row 20, 2000 receivers per shot
+    (every column of row 20).  True model: the three-layer rough-base model
+    (air in the top 100 rows, base at mean row 750, 10-node amplitude,
+    40-node correlation).  Starting model: the true one smoothed with a
+    20-node Gaussian (edge-replicated borders)."""
+    nz, nx = max(int(1000 * scale), 24), max(int(2000 * scale), 24)
+    nb = max(int(50 * scale), 4)
+    nt = nt or 4000
+    true = layered_2d(nz, nx, seed, air=max(int(100 * scale), 2), base_mean=int(750 * scale),
+                      amp=10 * scale, corr=max(40 * scale, 1.0))
+    sig = max(20 * scale, 1.0)
+    r = int(np.ceil(4 * sig))
+    start = _smooth(np.pad(true.astype(np.float64), r, mode="edge"), sig, (0, 1))[r:-r, r:-r]
+    row = max(int(20 * scale), 1)
+    cols = (np.arange(nshot) + 0.5) * nx / nshot
+    shots = [np.array([[c * H, row * H]], np.float32) for c in cols]
+    rec = np.stack([np.arange(nx) * H, np.full(nx, row * H)], 1).astype(np.float32)
+    par = dict(n=[nz, nx], o=[0.0, 0.0], d=[H, H], nt=nt, dt=DT, nb=nb, dabc=True,
+               free=False, cden=True, fdorder=4, expl=False)
+    return dict(kind="cart2d", par=par, true=true, start=start.astype(np.float32), shots=shots, rec=rec,
+                wav=ricker(nt, DT), ckpt=64, seed=seed)
