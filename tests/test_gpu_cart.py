@@ -92,3 +92,43 @@ def test_constant_density_flag_ignores_den(orc):
     tr, _, _ = awefd.run("cart2d", par, vel, sou, rec, wav, den=den)
     ref, _, _ = orc.run("cart2d", par, vel, sou, rec, wav)
     np.testing.assert_array_equal(tr, ref)
+
+
+def test_cart2d_graph_replays_setters_and_eager_path(orc):
+    """The first run of a 2-D loop launches step by step, a repeated run
+    captures the loop as one graph and later runs replay it: the direct run,
+    the capture, replays with other work in between, a setter (new sources),
+    a caller trace buffer and the profiled per-launch path all give the
+    oracle's traces."""
+    import ctypes as C
+
+    par, vel, sou, rec, wav = _case2d(nt=97)
+    ref, ufin, _ = orc.run("cart2d", par, vel, sou, rec, wav)
+    sou2 = np.array([[61.0, 52.0]], np.float32)
+    ref2, ufin2, _ = orc.run("cart2d", par, vel, sou2, rec, wav)
+    m = awefd.Model("cart2d", par)
+    m.set_model(vel)
+    m.set_sources(sou)
+    m.set_receivers(rec)
+    m.set_wavelet(wav)
+    for rep in range(3):
+        tr, _ = m.run()
+        np.testing.assert_array_equal(tr, ref, err_msg="replay %d" % rep)
+        np.testing.assert_array_equal(m.field(), ufin)
+        _ = awefd.run("cart2d", par, vel, sou2, rec, wav)      # other work between replays
+    m.set_sources(sou2)
+    tr, _ = m.run()
+    np.testing.assert_array_equal(tr, ref2)
+    np.testing.assert_array_equal(m.field(), ufin2)
+    m.profile(True)                                           # eager path
+    tr, _ = m.run()
+    m.profile(False)
+    np.testing.assert_array_equal(tr, ref2)
+    np.testing.assert_array_equal(m.field(), ufin2)
+    import torch
+
+    buf = torch.zeros((97, len(rec)), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    m.run_dev(buf.data_ptr())
+    np.testing.assert_array_equal(buf.cpu().numpy(), ref2)
+    m.close()

@@ -1,0 +1,51 @@
+"""FP64 error of the FP32 GPU loop, per measured workload.
+
+The GPU loop is bit-exact with the oracle's FP32 restatement of the
+reference; its distance from the oracle's FP64 run of the same scheme is the
+rounding error the whole-loop FP32 arithmetic carries.  Relative L2 of the
+traces and of the final padded field, printed per workload:
+
+  cart2d   the 2-D forward at full size (400 x 800, nb 50, nt 3000)
+  cart3d   the 3-D forward at a quarter of each axis (64 x 128 x 128, nt 600)
+  spher3d  the spherical shell at 0.15 of each axis (60 x 76 x 76, nt 300)
+  batch2d  one shot of the batched workload at full size (nt 2000)
+
+(The gradient's FP64 error is checked in test_gpu_adjoint.py.)  The numbers
+are reported, not targets: FP32 leapfrog rounding grows with the number of
+steps, and after 3000 steps the final field is the small residue the sponge
+leaves, so its relative error is the largest (about 1e-3 for cart2d).  The
+bound, 1e-2, only catches a scheme that drifts from the FP64 one."""
+import numpy as np
+import pytest
+
+from wavefield_imaging_amd import awefd, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def _cases():
+    b = synth.batch2d(nshot=1, nt=2000)
+    shot = dict(b["shots"][0], wav=b["wav"])
+    return [
+        ("cart2d", lambda: synth.cart2d()),
+        ("cart3d", lambda: synth.cart3d(scale=0.25, nt=600)),
+        ("spher3d", lambda: synth.spher3d(scale=0.15, nt=300)),
+        ("batch2d", lambda: shot),
+    ]
+
+
+@pytest.mark.parametrize("name", ["cart2d", "cart3d", "spher3d", "batch2d"])
+def test_fp64_error(orc, name):
+    case = dict(_cases())[name]()
+    kind = case["kind"]
+    print("fp64 error case %s %s nt %d" % (name, case["par"]["n"], case["par"]["nt"]), flush=True)
+    tr, _, field = awefd.run(kind, case["par"], case["vel"], case["sou"], case["rec"], case["wav"])
+    d64, f64, _ = orc.run(kind, case["par"], case["vel"], case["sou"], case["rec"], case["wav"], precision="f64")
+    et, ef = _rel(tr, d64), _rel(field, f64)
+    print("  %s: traces %.3e  final field %.3e" % (name, et, ef), flush=True)
+    assert np.abs(d64).max() > 0
+    assert et < 1e-2 and ef < 1e-2, (et, ef)

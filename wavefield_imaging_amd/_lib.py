@@ -53,6 +53,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise AwefdError("libawefd.so not built (run __graft_entry__.build()): %s" % LIB_PATH)
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7, but torch asks for it as libamdhip64.so).  Loaded
+    # first, it also serves libawefd's libamdhip64.so.7; loaded after
+    # libawefd, it would be a second runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     P = C.POINTER
     sig = {

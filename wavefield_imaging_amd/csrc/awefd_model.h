@@ -65,6 +65,18 @@ struct awefd_model {
 
     int steps_done;
 
+    /* 2-D time loop captured as one HIP graph (runs without snapshots or
+     * profiling): valid for model generation cgen and trace buffer cdtr */
+    hipGraph_t cgraph;
+    hipGraphExec_t cgexec;
+    unsigned cgen;
+    float *cdtr;
+    float *cPf, *cCf;             /* field roles after the captured loop      */
+    int cvalid;
+    unsigned eager_gen;           /* signature of the last eager 2-D run: the */
+    float *eager_dtr;             /* loop is captured when a run repeats it   */
+    int eager_seen;
+
     /* gradient state (awefd_adjoint.c): checkpoints of the last
      * awefd_run_ckpt, valid while gen is unchanged; setters bump gen */
     void *adj;
