@@ -58,6 +58,35 @@ __device__ __forceinline__ void znb(float c, float halo, float (&zm)[NOP], float
     }
 }
 
+/* Entry `lane` of a bucket slice (node and amplitude; lanes past the end
+ * get a node no lane matches): receivers carry r_n(receiver) * weight,
+ * sources w_n(column) * weight. */
+__device__ __forceinline__ void rec_entry(const awefd_adj2d_args &a, const float *res, int b0, int cnt, int lane,
+                                          int &z, int &x, float &amp)
+{
+    z = x = -1;
+    amp = 0.f;
+    if (lane < cnt) {
+        const awefd_inj en = a.rinj[b0 + lane];
+        z = en.z;
+        x = en.x;
+        amp = res[en.src] * en.w;
+    }
+}
+
+__device__ __forceinline__ void src_entry(const awefd_adj2d_args &a, int n, int b0, int cnt, int lane, int &z, int &x,
+                                          float &amp)
+{
+    z = x = -1;
+    amp = 0.f;
+    if (lane < cnt) {
+        const awefd_inj en = a.sinj[b0 + lane];
+        z = en.z;
+        x = en.x;
+        amp = a.wav[(long)n * a.wstride + en.src] * en.w;
+    }
+}
+
 /* One backward step n of one grid: alpha_n into A[n % 3], dJ/dV
  * accumulated. */
 template <int NOP, int RX>
@@ -84,12 +113,29 @@ __device__ __forceinline__ void adj_body(const awefd_adj2d_args &a, int bid, int
     const unsigned vzh = zhv ? (unsigned)zz * 4u : OOB;
     const bool zi = iz >= NOP && iz < nz - NOP;
     const bool zhi = zz >= NOP && zz < nz - NOP;
-    const float gz = (a.dabc && zv) ? a.sp1[iz] : 1.f;
-    const float gzh = (a.dabc && zhv) ? a.sp1[zz] : 1.f;
+    /* sponge factors (tables of ones without a sponge), read at clamped
+     * indices so that no load waits on a branch; lanes and rows outside the
+     * grid never use theirs */
+    const float gz = a.sp1[min(iz, nz - 1)];
+    const float gzh = a.sp1[min(max(zz, 0), nz - 1)];
+    float gxv[H];
+#pragma unroll
+    for (int r = 0; r < H; r++) gxv[r] = a.sp2[min(max(x0 - NOP + r, 0), nx - 1)];
     const unsigned rowb = (unsigned)a.ld * 4u;
     const unsigned nbytes = (unsigned)a.ld * (unsigned)nx * 4u;
     const rsrc_t rA1 = mk_rsrc(A1, nbytes), rA2 = mk_rsrc(A2, nbytes), rV = mk_rsrc(a.V, nbytes);
     const rsrc_t rC = mk_rsrc(Cn, nbytes), rA0 = mk_rsrc(A0, nbytes), rG = mk_rsrc(a.gV, nbytes);
+
+
+    /* the own rows' alpha_{n+2} and dJ/dV first: every load is in flight
+     * before the first store (buffer accesses keep program order) */
+    float a2v[RX], g0[RX];
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const unsigned so = (unsigned)(x0 + j) * rowb;   /* rows >= nx: beyond the range */
+        a2v[j] = bld(rA2, vz, so);
+        g0[j] = bld(rG, vz, so);
+    }
 
     /* y = V M G alpha_{n+1} and c_n on the RX rows and NOP halo rows each
      * side; rows outside the grid load zeros (offsets beyond the range) */
@@ -99,7 +145,7 @@ __device__ __forceinline__ void adj_body(const awefd_adj2d_args &a, int bid, int
         const int xr = x0 - NOP + r;
         const unsigned so = (unsigned)xr * rowb;
         const bool xi = xr >= NOP && xr < nx - NOP;
-        const float gx = (a.dabc && xr >= 0 && xr < nx) ? a.sp2[xr] : 1.f;
+        const float gx = gxv[r];
         const float a1 = bld(rA1, vz, so), v = bld(rV, vz, so);
         const float gg = gz * gx;
         const float tt = gg * a1;
@@ -114,6 +160,17 @@ __device__ __forceinline__ void adj_body(const awefd_adj2d_args &a, int bid, int
             chl[j] = bld(rC, vzh, so);
         }
     }
+
+    /* receiver and injection entries of this tile column, one per lane
+     * (64 at a time): the first 64 requested behind the field loads, so
+     * that their latency hides under the stencil work */
+    const int col = tzi + a.ntz * txi;
+    const int rb0 = a.rbeg[col], rn = a.rbeg[col + 1] - rb0;
+    const int sb0 = a.sbeg[col], sn = a.sbeg[col + 1] - sb0;
+    int rz, rx, sz, sx;
+    float ramp, samp;
+    rec_entry(a, res, rb0, rn, lane, rz, rx, ramp);
+    src_entry(a, n, sb0, sn, lane, sz, sx, samp);
 
     float al[RX], gv[RX];
 #pragma unroll
@@ -131,34 +188,35 @@ __device__ __forceinline__ void adj_body(const awefd_adj2d_args &a, int bid, int
 #pragma unroll
         for (int k = 1; k <= NOP; k++)
             lc = lc + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]) + a.cx[k - 1] * (c[r + k] + c[r - k]);
-        const unsigned so = (unsigned)ix * rowb;
-        const float tb = g[j] * bld(rA2, vz, so);       /* t_{n+1} = G alpha_{n+2} */
+        const float tb = g[j] * a2v[j];                 /* t_{n+1} = G alpha_{n+2} */
         al[j] = (inter ? 2.f * t[j] : 0.f) + ly + g[j] * (inter ? -tb : tb);
         gv[j] = inter ? lc * t[j] : 0.f;
     }
 
-    /* receivers (G R^T r_n) and the injection term, from this tile column's
-     * buckets, in point order */
-    const int col = tzi + a.ntz * txi;
-    for (int e = a.rbeg[col]; e < a.rbeg[col + 1]; e++) {
-        const awefd_inj en = a.rinj[e];
-        const float amp = res[en.src] * en.w;
+    /* receivers (G R^T r_n) and the injection term, entry after entry in
+     * bucket order, broadcast from the lane that loaded it */
+    for (int q = 0; q < rn; q++) {
+        if (q && (q & (WAVE - 1)) == 0) rec_entry(a, res, rb0 + q, rn - q, lane, rz, rx, ramp);
+        const int l = q & (WAVE - 1);
+        const int qz = __builtin_amdgcn_readlane(rz, l), qx = __builtin_amdgcn_readlane(rx, l);
+        const float qa = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, ramp), l));
 #pragma unroll
-        for (int j = 0; j < RX; j++) al[j] = (en.z == iz && en.x == x0 + j) ? al[j] + g[j] * amp : al[j];
+        for (int j = 0; j < RX; j++) al[j] = (qz == iz && qx == x0 + j) ? al[j] + g[j] * qa : al[j];
     }
-    for (int e = a.sbeg[col]; e < a.sbeg[col + 1]; e++) {
-        const awefd_inj en = a.sinj[e];
-        const float amp = a.wav[(long)n * a.wstride + en.src] * en.w;
+    for (int q = 0; q < sn; q++) {
+        if (q && (q & (WAVE - 1)) == 0) src_entry(a, n, sb0 + q, sn - q, lane, sz, sx, samp);
+        const int l = q & (WAVE - 1);
+        const int qz = __builtin_amdgcn_readlane(sz, l), qx = __builtin_amdgcn_readlane(sx, l);
+        const float qa = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, samp), l));
 #pragma unroll
-        for (int j = 0; j < RX; j++) gv[j] = (en.z == iz && en.x == x0 + j) ? gv[j] + amp * t[j] : gv[j];
+        for (int j = 0; j < RX; j++) gv[j] = (qz == iz && qx == x0 + j) ? gv[j] + qa * t[j] : gv[j];
     }
 
 #pragma unroll
     for (int j = 0; j < RX; j++) {
         const unsigned so = (unsigned)(x0 + j) * rowb;   /* rows >= nx: beyond the range */
         bst(rA0, vz, so, al[j]);
-        const float g0 = bld(rG, vz, so);
-        bst(rG, vz, so, g0 + gv[j]);
+        bst(rG, vz, so, g0[j] + gv[j]);
     }
 }
 

@@ -45,6 +45,7 @@ typedef struct {
     unsigned lists_gen;
     int lists_ok;
     awefd_adj2d_args *d_args; /* device copy of the kernel arguments    */
+    float *ones;              /* sponge table of ones (runs without dabc) */
 } adj_state;
 
 static void free_lists(adj_state *s)
@@ -62,6 +63,7 @@ void awefd__adjoint_free(awefd_model *m)
     hipSetDevice(m->dev);
     hipFree(s->ck); hipFree(s->seg);
     hipFree(s->A[0]); hipFree(s->A[1]); hipFree(s->A[2]); hipFree(s->gV); hipFree(s->d_args);
+    hipFree(s->ones);
     free_lists(s);
     free(s);
     m->adj = NULL;
@@ -239,6 +241,12 @@ int awefd__adj_prepare(awefd_model *m)
     if (s->K > 1) rc = dev_alloc(&s->seg, fb * (s->K - 1));
     for (int i = 0; i < 3 && !rc; i++) rc = dev_alloc(&s->A[i], fb);
     if (!rc) rc = dev_alloc(&s->gV, fb);
+    if (!rc && !m->p.dabc && !s->ones) {
+        const long n = m->np[0] > m->np[1] ? m->np[0] : m->np[1];
+        rc = dev_alloc(&s->ones, sizeof(float) * n);
+        if (!rc && awefd_launch_fill(s->ones, n, 1.f, m->stream))
+            rc = awefd__fail(AWEFD_EHIP, "gradient sponge table");
+    }
     if (!rc && !s->d_args) {
         hipError_t e = hipMalloc((void **)&s->d_args, sizeof(awefd_adj2d_args));
         if (e != hipSuccess) rc = awefd__fail(AWEFD_ENOMEM, "gradient arguments: %s", hipGetErrorString(e));
@@ -277,8 +285,8 @@ void awefd__adj_args(const awefd_model *m, const float *d_res, awefd_adj2d_args 
     g->nb = m->nb;
     g->dabc = m->p.dabc;
     g->idt2 = (float)(1.0 / ((double)m->p.dt * m->p.dt));
-    g->sp1 = m->dsp[0];
-    g->sp2 = m->dsp[1];
+    g->sp1 = m->p.dabc ? m->dsp[0] : s->ones;
+    g->sp2 = m->p.dabc ? m->dsp[1] : s->ones;
     g->c0 = m->c2[0];
     for (int k = 1; k <= m->nop; k++) {
         g->cz[k - 1] = m->c2[k];

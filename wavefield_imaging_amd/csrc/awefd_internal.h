@@ -95,7 +95,7 @@ typedef struct {
     float *gout;             /* dJ/ds output of the chain (model grid)        */
     int nz, nx, ld, mz, mx, nb, dabc;
     float idt2;              /* 1/dt^2                                        */
-    const float *sp1, *sp2;  /* sponge factors per axis                       */
+    const float *sp1, *sp2;  /* sponge factors per axis (ones without dabc)   */
     float c0, cz[AWEFD_MAX_NOP], cx[AWEFD_MAX_NOP];
     int ntz, ntx, nblk;      /* 64 x AWEFD_ADJ_TX tiles                       */
     /* receiver and injection nodes, bucketed per tile column like the
