@@ -116,10 +116,22 @@ __device__ __forceinline__ unsigned soff(int x, int y, int ybase, unsigned rowb,
     return (unsigned)x * rowb + (unsigned)(y - ybase) * psz;
 }
 
+/* x halo row h of a wave (h < NOP: x0-NOP+h, h >= NOP: x0+RX+h-NOP) lies
+ * in the block's own rows [bx0, bx0 + TX): then it comes from LDS */
+template <int NOP, int RX>
+__device__ __forceinline__ bool halo_in_block(int h, int x0, int bx0, int tx)
+{
+    const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
+    return hx >= bx0 && hx < bx0 + tx;
+}
+
+/* Loads of one plane: previous field, (v dt)^2 and the z-halo value of the
+ * wave's rows, and the x-halo rows outside the block (the others are
+ * exchanged through LDS; their loads get an out-of-range offset). */
 template <int NOP, int RX>
 __device__ __forceinline__ void load_pvh(float (&p)[RX], float (&v)[RX], float (&zh)[RX],
                                          float (&hal)[2 * NOP], rsrc_t rP, rsrc_t rV, rsrc_t rC,
-                                         unsigned vz, unsigned vzh, int x0, int y, int ybase,
+                                         unsigned vz, unsigned vzh, int x0, int bx0, int tx, int y, int ybase,
                                          unsigned rowb, unsigned psz)
 {
 #pragma unroll
@@ -132,7 +144,7 @@ __device__ __forceinline__ void load_pvh(float (&p)[RX], float (&v)[RX], float (
 #pragma unroll
     for (int h = 0; h < 2 * NOP; h++) {
         const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
-        hal[h] = bld(rC, vz, soff(hx, y, ybase, rowb, psz));
+        hal[h] = bld(rC, halo_in_block<NOP, RX>(h, x0, bx0, tx) ? OOB : vz, soff(hx, y, ybase, rowb, psz));
     }
 }
 
@@ -270,7 +282,11 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
 #pragma unroll
         for (int j = 0; j < RX; j++) q[j][0] = bld(rC, vz, soff(x0 + j, 0, 0, rowb, psz));
     }
-    load_pvh<NOP, RX>(bp_[0], bv_[0], bz_[0], bh_[0], rP, rV, rC, vz, vzh, x0, y0, ybase, rowb, psz);
+    const int bx0 = txi * TX;                            /* the block's first row */
+    load_pvh<NOP, RX>(bp_[0], bv_[0], bz_[0], bh_[0], rP, rV, rC, vz, vzh, x0, bx0, TX, y0, ybase, rowb, psz);
+    /* the centre plane's rows of every wave of the block, for the x
+     * neighbours inside the block (two planes, one barrier per plane) */
+    __shared__ float xrow[2][TX][WAVE];
     load_den<VD, RX>(bg_[0], rR, vz4, x0, y0, ybase, rowb, psz);
 
     /* injection entries of this tile column, restricted to planes >= y0 */
@@ -297,13 +313,24 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
                     for (int j = 0; j < RX; j++)
                         q[j][(S0 + 2 * NOP + 1) % QN] = bld(rC, vz, soff(x0 + j, y + 1 + NOP, ybase, rowb, psz));
                 }
-                load_pvh<NOP, RX>(bp_[B ^ 1], bv_[B ^ 1], bz_[B ^ 1], bh_[B ^ 1], rP, rV, rC, vz, vzh, x0,
+                load_pvh<NOP, RX>(bp_[B ^ 1], bv_[B ^ 1], bz_[B ^ 1], bh_[B ^ 1], rP, rV, rC, vz, vzh, x0, bx0, TX,
                               y + 1, ybase, rowb, psz);
                 load_den<VD, RX>(bg_[B ^ 1], rR, vz4, x0, y + 1, ybase, rowb, psz);
             }
             float cen[RX];
 #pragma unroll
             for (int j = 0; j < RX; j++) cen[j] = NDIM == 3 ? q[j][(S0 + NOP) % QN] : q[j][0];
+            /* x halo rows inside the block from the other waves' centre rows */
+#pragma unroll
+            for (int j = 0; j < RX; j++) xrow[B][wv * RX + j][lane] = cen[j];
+            __syncthreads();
+            float hal[2 * NOP];
+#pragma unroll
+            for (int h = 0; h < 2 * NOP; h++) {
+                const int hr = h < NOP ? wv * RX - NOP + h : wv * RX + RX + (h - NOP);
+                const float l = xrow[B][min(max(hr, 0), TX - 1)][lane];
+                hal[h] = halo_in_block<NOP, RX>(h, x0, bx0, TX) ? l : bh_[B][h];
+            }
 
             const bool yint = NDIM == 2 || (y >= NOP && y < n3 - NOP);
             const bool ysp = NDIM == 3 && a.dabc && (y < a.nb || y >= n3 - a.nb);
@@ -341,25 +368,25 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
                 if (SSE) {
 #pragma unroll
                     for (int k = 1; k <= NOP; k++) {
-                        const float xpk = xnb<NOP, RX>(cen, bh_[B], j, k), xmk = xnb<NOP, RX>(cen, bh_[B], j, -k);
+                        const float xpk = xnb<NOP, RX>(cen, hal, j, k), xmk = xnb<NOP, RX>(cen, hal, j, -k);
                         lap = lap + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]);
                         lap = lap + a.cx[k - 1] * (xpk + xmk);
                         if (NDIM == 3)
                             lap = lap + a.cy[k - 1] * (q[j][(S0 + NOP + k) % QN] + q[j][(S0 + NOP - k) % QN]);
                     }
-                    if (VD) lap = den_term<NOP, NDIM, SSE>(lap, zp, zm, cen, bh_[B], q[j], S0, j, bg_[B][j], a);
+                    if (VD) lap = den_term<NOP, NDIM, SSE>(lap, zp, zm, cen, hal, q[j], S0, j, bg_[B][j], a);
                     const float t = 2.f * c - p;
                     v = t + bv_[B][j] * lap;
                 } else {
 #pragma unroll
                     for (int k = 1; k <= NOP; k++) {
-                        const float xpk = xnb<NOP, RX>(cen, bh_[B], j, k), xmk = xnb<NOP, RX>(cen, bh_[B], j, -k);
+                        const float xpk = xnb<NOP, RX>(cen, hal, j, k), xmk = xnb<NOP, RX>(cen, hal, j, -k);
                         float t = (zm[k - 1] + zp[k - 1]) * a.cz[k - 1] + (xmk + xpk) * a.cx[k - 1];
                         if (NDIM == 3)
                             t = t + (q[j][(S0 + NOP - k) % QN] + q[j][(S0 + NOP + k) % QN]) * a.cy[k - 1];
                         lap = lap + t;
                     }
-                    if (VD) lap = den_term<NOP, NDIM, SSE>(lap, zp, zm, cen, bh_[B], q[j], S0, j, bg_[B][j], a);
+                    if (VD) lap = den_term<NOP, NDIM, SSE>(lap, zp, zm, cen, hal, q[j], S0, j, bg_[B][j], a);
                     const float vl = bv_[B][j] * lap;
                     v = (float)(2.0 * (double)c - (double)p + (double)vl);
                 }
