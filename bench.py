@@ -146,8 +146,9 @@ def measure(workload, nt, steps, warmup):
 
 def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
     """256 independent 2-D shots of uneven size (nz ~ U[200,600], nx ~
-    U[400,1200], nb=50): the batched graph-captured run (autotuned tile) and
-    the plain per-shot loop of the single-grid kernel, in shots/s."""
+    U[400,1200], nb=50): the batched graph-captured run (autotuned tile,
+    shot grouping and steps per launch) and the plain per-shot loop of the
+    single-grid kernel, in shots/s."""
     import numpy as np
 
     from wavefield_imaging_amd import awefd, synth
@@ -164,7 +165,7 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
     cells = int(sum(np.prod(m.padded) for m in ms))
     print("bench: batch2d %d shots, %.3g padded cells, nt=%d" % (nshot, cells, nt), file=sys.stderr, flush=True)
     b = awefd.Batch(ms)
-    rx, grp, tune_ms = b.tune(steps=tune_steps)
+    rx, grp, depth, tune_ms = b.tune_depth(steps=tune_steps)
     b.run()                                   # warm-up (graph capture + launch)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -174,8 +175,8 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
     out = dict(workload="batch2d %d shots nz~U[200,600] nx~U[400,1200] nb=50 nt=%d, %d padded cells"
                         % (nshot, nt, cells),
                shots_per_s=round(batched, 3), gcells_per_s=round(cells * nt * steps / (t1 - t0) / 1e9, 3),
-               s_per_batch=round((t1 - t0) / steps, 4), tile_rx=rx, group_mb=grp,
-               tune_ms_per_step={"rx%d_g%d" % k: round(v, 4) for k, v in tune_ms.items()})
+               s_per_batch=round((t1 - t0) / steps, 4), tile_rx=rx, group_mb=grp, steps_per_launch=depth,
+               tune_ms_per_step={"d%d_rx%d_g%d" % k: round(v, 4) for k, v in tune_ms.items()})
     b.close()
     if per_shot:
         # warm-up of the single-grid kernel, on the per-launch path (a

@@ -105,3 +105,64 @@ def test_batch_scalar_order_and_free_surface(orc):
     b.close()
     for m in ms:
         m.close()
+
+
+@pytest.mark.parametrize("nt,fdorder,kw", [
+    (61, 4, {}), (60, 4, {}), (59, 8, {}), (62, 12, {}), (57, 4, dict(sse=False)),
+    (58, 6, dict(free=True)), (55, 4, dict(optfd=True)),
+])
+def test_two_steps_per_pass_match_single_runs(orc, nt, fdorder, kw):
+    """Depth 2 (two time steps per launch on overlapping tiles, the first
+    step's field through LDS): every shot's traces and final field equal its
+    own single-step run bit for bit, for odd and even nt, every stencil
+    order, both operation orders, one and several shot groups, and on graph
+    replays."""
+    shots = _shots(6, nt, seed=11, fdorder=fdorder, **kw)
+    print("depth-2 batch nt", nt, "fdorder", fdorder, kw, flush=True)
+    ref = _single(shots)
+    ms = _models(shots)
+    b = awefd.Batch(ms)
+    for rx, grp in ((4, 0), (8, 1), (4, 1)):
+        b.set_tile(rx, grp)
+        b.set_depth(2)
+        for rep in range(2):
+            b.run()
+            for s, m in enumerate(ms):
+                np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg="shot %d rx %d rep %d" % (s, rx, rep))
+                np.testing.assert_array_equal(m.field(), ref[s][1], err_msg="shot %d rx %d rep %d" % (s, rx, rep))
+    tr, fin, _ = orc.run("cart2d", *shots[0])
+    np.testing.assert_array_equal(ref[0][0], tr)
+    b.close()
+    for m in ms:
+        m.close()
+
+
+def test_two_steps_sources_on_tile_edges_and_tuning(orc):
+    """Several sources per shot, some on the overlap of neighbouring tiles
+    (each tile injects into its whole first-step region), expl=y; then the
+    depth-aware autotuner, whose choice must give the same results."""
+    shots = _shots(4, 40, seed=21, expl=True)
+    out = []
+    for par, vel, sou, rec, wav in shots:
+        nz, nx = par["n"]
+        xs = np.array([11, 12, 23, 24, 35, 36, 47]) % nx
+        sou = np.stack([xs * 5.0 + 0.5, np.full(len(xs), (nz // 2) * 5.0 + 2.0)], 1).astype(np.float32)
+        out.append((par, vel, sou, rec, wav))
+    ref = _single(out)
+    ms = _models(out)
+    b = awefd.Batch(ms)
+    b.set_tile(4, 0)
+    b.set_depth(2)
+    b.run()
+    for s, m in enumerate(ms):
+        np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg="shot %d" % s)
+    rx, grp, depth, times = b.tune_depth(steps=10)
+    print("tuned rx", rx, "group", grp, "depth", depth, flush=True)
+    assert depth in (1, 2) and all(t > 0 for t in times.values())
+    b.run()
+    for s, m in enumerate(ms):
+        np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg="tuned shot %d" % s)
+        np.testing.assert_array_equal(m.field(), ref[s][1], err_msg="tuned shot %d" % s)
+    b.close()
+    for m in ms:
+        m.close()

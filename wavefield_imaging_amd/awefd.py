@@ -216,6 +216,20 @@ class Batch:
     def run(self):
         L.check(self.lib.awefd_batch_run(self.h))
 
+    def set_depth(self, depth):
+        """Steps per launch: 1, or 2 (tile rx 4 or 8)."""
+        L.check(self.lib.awefd_batch_set_depth(self.h, int(depth)))
+
+    def tune_depth(self, steps=20):
+        """Time every (depth, tile, group) choice and keep the fastest;
+        returns (rx, group_mb, depth, {(depth, rx, group_mb): ms per step})."""
+        rx, grp, dep = C.c_int(), C.c_int(), C.c_int()
+        n = len(self.RX) * len(self.GROUP_MB)
+        ms = (C.c_double * (2 * n))()
+        L.check(self.lib.awefd_batch_tune_depth(self.h, int(steps), C.byref(rx), C.byref(grp), C.byref(dep), ms))
+        keys = [(d, r, g) for d in (1, 2) for r in self.RX for g in self.GROUP_MB]
+        return rx.value, grp.value, dep.value, {k: v for k, v in zip(keys, list(ms)) if v > 0}
+
     def run_ckpt(self, ckpt=0):
         """Every shot's forward loop keeping checkpoints every ``ckpt`` steps
         (0: ceil(sqrt(nt))) for adjoint_grad."""

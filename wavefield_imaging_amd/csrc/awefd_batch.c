@@ -72,6 +72,19 @@ struct awefd_batch {
     long *d_zcbeg, znch;
     long abshot_cap;
     float **fwd_ck;                  /* checkpoint bases the forward graph holds */
+    /* two time steps per pass (depth 2) */
+    int depth;                       /* 1 or 2                                   */
+    float **e0, **e1;                /* per shot: the extra field pair           */
+    int *d_bshot2;                   /* two-step launch: block -> shot           */
+    long *goff2;
+    int *gnblk2;
+    awefd_inj **d_inj2;              /* per shot: two-step injection buckets     */
+    int **d_ibeg2;
+    int t2_ok;                       /* two-step tables built for the tile       */
+    awefd_copy_job *d_fjobs;         /* end of a depth-2 run: fields back into   */
+    long *d_fcbeg, fnch;             /* the models' own buffers                  */
+    int nfjob, fin_p, fin_c;
+    double tune2_ms[4][NGC];
     hipGraph_t gfwd, gbwd;
     hipGraphExec_t xfwd, xbwd;
     int fwd_ok, bwd_ok;
@@ -109,6 +122,21 @@ void awefd_batch_destroy(awefd_batch *b)
     free(b->grp);
     free(b->h);
     free(b->fwd_ck);
+    for (int s = 0; s < b->nshot; s++) {
+        if (b->e0) dfree(&b->e0[s]);
+        if (b->e1) dfree(&b->e1[s]);
+        if (b->d_inj2) dfree(&b->d_inj2[s]);
+        if (b->d_ibeg2) dfree(&b->d_ibeg2[s]);
+    }
+    free(b->e0);
+    free(b->e1);
+    free(b->d_inj2);
+    free(b->d_ibeg2);
+    dfree(&b->d_bshot2);
+    dfree(&b->d_fjobs);
+    dfree(&b->d_fcbeg);
+    free(b->goff2);
+    free(b->gnblk2);
     for (int s = 0; s < b->nshot; s++) {
         if (b->d_inj) dfree(&b->d_inj[s]);
         if (b->d_ibeg) dfree(&b->d_ibeg[s]);
@@ -282,6 +310,8 @@ static int batch_build(awefd_batch *b, int rx, int group_mb)
     }
     /* captured graphs refer to the tables just replaced */
     b->graph_ok = b->fwd_ok = b->bwd_ok = 0;
+    b->t2_ok = 0;
+    if (!rc && b->depth == 2 && rx != 4 && rx != 8) b->depth = 1;
     return rc;
 }
 
@@ -305,6 +335,251 @@ static int batch_steps(awefd_batch *b, int nsteps)
                                              b->rx, b->stream);
             if (e) return awefd__fail(AWEFD_EHIP, "batch step %d: %s", it, hipGetErrorString((hipError_t)e));
         }
+    return AWEFD_OK;
+}
+
+/* ---- Two time steps per pass (depth 2) ----
+ * Tiles of (64 - 2 nop) z x (4 rx - 2 nop) x outputs, each computing its
+ * first step on 64 x 4 rx nodes; the shot's four field buffers rotate
+ * (u0, u1 and the batch's e0, e1): a pass reads the previous and current
+ * field and writes the next two into the other pair. */
+
+static int t2_tile(const awefd_batch *b, int *sz, int *sx)
+{
+    *sz = 64 - 2 * b->nop;
+    *sx = AWEFD_BATCH_NWAVE * b->rx - 2 * b->nop;
+    return (b->rx == 4 || b->rx == 8) && *sx > 0;
+}
+
+/* every (source node, tile) pair whose tile's first-step region holds the
+ * node, bucketed per tile */
+static int t2_buckets(const awefd_batch *b, const awefd_model *m, int ntz2, int ntx2, awefd_inj **inj, int **ibeg,
+                      long *ninj)
+{
+    int sz, sx;
+    t2_tile(b, &sz, &sx);
+    const int rows = AWEFD_BATCH_NWAVE * b->rx, nop = b->nop;
+    long cap = 16L * 4 * (m->ns > 0 ? m->ns : 1), n = 0;
+    awefd_inj_key *keys = malloc(sizeof(*keys) * cap);
+    int *beg = malloc(sizeof(int) * (ntz2 * ntx2 + 1));
+    if (!keys || !beg) {
+        free(keys); free(beg);
+        return awefd__fail(AWEFD_ENOMEM, "two-step injection buckets");
+    }
+    for (int s = 0; s < m->ns; s++)
+        for (int q = 0; q < 4; q++) {
+            const int z = m->h_sj[3 * s] + (q & 1), x = m->h_sj[3 * s + 1] + ((q >> 1) & 1);
+            for (int tz = 0; tz < ntz2; tz++) {
+                if (z < tz * sz - nop || z >= tz * sz - nop + 64) continue;
+                for (int tx = 0; tx < ntx2; tx++) {
+                    if (x < tx * sx - nop || x >= tx * sx - nop + rows) continue;
+                    if (n == cap) {
+                        awefd_inj_key *k2 = realloc(keys, sizeof(*keys) * cap * 2);
+                        if (!k2) {
+                            free(keys); free(beg);
+                            return awefd__fail(AWEFD_ENOMEM, "two-step injection buckets");
+                        }
+                        keys = k2;
+                        cap *= 2;
+                    }
+                    awefd_inj_key *k = &keys[n++];
+                    k->e.z = z;
+                    k->e.x = x;
+                    k->e.y = 0;
+                    k->e.src = s;
+                    k->e.w = m->h_sw[8 * s + q];
+                    k->bucket = tz + ntz2 * tx;
+                }
+            }
+        }
+    awefd_inj *e = malloc(sizeof(*e) * (n > 0 ? n : 1));
+    if (!e) {
+        free(keys); free(beg);
+        return awefd__fail(AWEFD_ENOMEM, "two-step injection buckets");
+    }
+    awefd_sort_inj(keys, n, ntz2 * ntx2, e, beg);
+    free(keys);
+    *inj = e;
+    *ibeg = beg;
+    *ninj = n;
+    return AWEFD_OK;
+}
+
+/* Two-step tables for the current tile and groups. */
+static int t2_build(awefd_batch *b)
+{
+    int sz, sx;
+    if (!t2_tile(b, &sz, &sx))
+        return awefd__fail(AWEFD_EINVAL, "two steps per pass need tile rx 4 or 8 (and 4 rx > 2 nop)");
+    const int nthr = AWEFD_BATCH_NWAVE * 64, ns = b->nshot, ng = b->ngroup;
+    if (!b->e0) b->e0 = calloc(ns, sizeof(float *));
+    if (!b->e1) b->e1 = calloc(ns, sizeof(float *));
+    if (!b->d_inj2) b->d_inj2 = calloc(ns, sizeof(awefd_inj *));
+    if (!b->d_ibeg2) b->d_ibeg2 = calloc(ns, sizeof(int *));
+    long *goff2 = calloc(ng + 1, sizeof(long));
+    int *gnblk2 = calloc(ng, sizeof(int)), *nb = malloc(sizeof(int) * ns);
+    int rc = (!b->e0 || !b->e1 || !b->d_inj2 || !b->d_ibeg2 || !goff2 || !gnblk2 || !nb)
+                 ? awefd__fail(AWEFD_ENOMEM, "two-step tables") : AWEFD_OK;
+    for (int s = 0; s < ns && !rc; s++) {
+        awefd_model *m = b->m[s];
+        awefd_cart_args *h = &b->h[s];
+        hipError_t e = hipSuccess;
+        if (!b->e0[s]) e = hipMalloc((void **)&b->e0[s], sizeof(float) * m->nalloc);
+        if (e == hipSuccess && !b->e1[s]) e = hipMalloc((void **)&b->e1[s], sizeof(float) * m->nalloc);
+        if (e != hipSuccess) {
+            rc = awefd__fail(AWEFD_ENOMEM, "two-step fields: %s", hipGetErrorString(e));
+            break;
+        }
+        h->F[0] = b->u0[s];
+        h->F[1] = b->u1[s];
+        h->F[2] = b->e0[s];
+        h->F[3] = b->e1[s];
+        h->ntz2 = (m->np[0] + sz - 1) / sz;
+        h->ntx2 = (m->np[1] + sx - 1) / sx;
+        h->nblk2 = h->ntz2 * h->ntx2;
+        nb[s] = h->nblk2 + (m->nr + nthr - 1) / nthr;
+        h->blk02 = gnblk2[b->grp[s]];
+        if ((long)gnblk2[b->grp[s]] + nb[s] > 0x7fffffffL) rc = awefd__fail(AWEFD_EINVAL, "batch: too many blocks");
+        gnblk2[b->grp[s]] += nb[s];
+        awefd_inj *inj = NULL;
+        int *ibeg = NULL;
+        long ninj = 0;
+        if (!rc) rc = t2_buckets(b, m, h->ntz2, h->ntx2, &inj, &ibeg, &ninj);
+        if (!rc) {
+            dfree(&b->d_inj2[s]);
+            dfree(&b->d_ibeg2[s]);
+            e = hipMalloc((void **)&b->d_inj2[s], sizeof(*inj) * (ninj > 0 ? ninj : 1));
+            if (e == hipSuccess) e = hipMalloc((void **)&b->d_ibeg2[s], sizeof(int) * (h->nblk2 + 1));
+            if (e == hipSuccess && ninj > 0) e = hipMemcpy(b->d_inj2[s], inj, sizeof(*inj) * ninj, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(b->d_ibeg2[s], ibeg, sizeof(int) * (h->nblk2 + 1), hipMemcpyHostToDevice);
+            if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "two-step injection upload: %s", hipGetErrorString(e));
+            h->inj2 = b->d_inj2[s];
+            h->ibeg2 = b->d_ibeg2[s];
+        }
+        free(inj);
+        free(ibeg);
+    }
+    for (int g = 0; g < ng && !rc; g++) goff2[g + 1] = goff2[g] + gnblk2[g];
+    int *bshot = rc ? NULL : malloc(sizeof(int) * (goff2[ng] > 0 ? goff2[ng] : 1));
+    if (!rc && !bshot) rc = awefd__fail(AWEFD_ENOMEM, "two-step block table");
+    for (int s = 0; s < ns && !rc; s++)
+        for (int k = 0; k < nb[s]; k++) bshot[goff2[b->grp[s]] + b->h[s].blk02 + k] = s;
+    if (!rc) {
+        dfree(&b->d_bshot2);
+        hipError_t e = hipMalloc((void **)&b->d_bshot2, sizeof(int) * goff2[ng]);
+        if (e == hipSuccess) e = hipMemcpy(b->d_bshot2, bshot, sizeof(int) * goff2[ng], hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_shots, b->h, sizeof(*b->h) * ns, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "two-step table upload: %s", hipGetErrorString(e));
+    }
+    free(bshot);
+    free(nb);
+    if (!rc) {
+        free(b->goff2);
+        free(b->gnblk2);
+        b->goff2 = goff2;
+        b->gnblk2 = gnblk2;
+        b->t2_ok = 1;
+    } else {
+        free(goff2);
+        free(gnblk2);
+    }
+    return rc;
+}
+
+/* nsteps steps of group g in passes of two steps, then one or two single
+ * steps (each pass extracts the rows of its two input fields, so ending on
+ * single steps extracts every row); final roles of the previous and current
+ * field in *fp, *fc */
+static int t2_steps(awefd_batch *b, int g, int nsteps, int *fp, int *fc)
+{
+    int iP = 0, iC = 1, it = 0;
+    const int npair = (nsteps - 1) / 2;
+    for (int k = 0; k < npair; k++, it += 2) {
+        int d[2], nd = 0;
+        for (int i = 0; i < 4; i++)
+            if (i != iP && i != iC) d[nd++] = i;
+        const int roles = iP | iC << 2 | d[0] << 4 | d[1] << 6;
+        int e = awefd_launch_cart_batch_t2(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it, roles, 1, b->nop,
+                                           b->sse, b->rx, b->stream);
+        if (e) return awefd__fail(AWEFD_EHIP, "two-step pass %d: %s", it, hipGetErrorString((hipError_t)e));
+        iP = d[0];
+        iC = d[1];
+    }
+    for (; it < nsteps; it++) {
+        int d0 = 0;
+        while (d0 == iP || d0 == iC) d0++;
+        const int roles = iP | iC << 2 | d0 << 4 | d0 << 6;
+        int e = awefd_launch_cart_batch_t2(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it, roles, 0, b->nop,
+                                           b->sse, b->rx, b->stream);
+        if (e) return awefd__fail(AWEFD_EHIP, "one-step pass %d: %s", it, hipGetErrorString((hipError_t)e));
+        iP = iC;
+        iC = d0;
+    }
+    *fp = iP;
+    *fc = iC;
+    return AWEFD_OK;
+}
+
+/* After a depth-2 run the last two fields may sit in the batch's extra pair:
+ * copies into the models' own buffers (the slot the other field does not
+ * occupy) */
+static int t2_final_jobs(awefd_batch *b, int fp, int fc)
+{
+    int tp = fp, tc = fc;
+    if (tp >= 2) tp = tc == 0 ? 1 : 0;
+    if (tc >= 2) tc = tp == 0 ? 1 : 0;
+    b->fin_p = tp;
+    b->fin_c = tc;
+    const int per = (fp >= 2) + (fc >= 2);
+    b->nfjob = per * b->nshot;
+    dfree(&b->d_fjobs);
+    dfree(&b->d_fcbeg);
+    if (!per) return AWEFD_OK;
+    awefd_copy_job *jobs = malloc(sizeof(*jobs) * b->nfjob);
+    long *cbeg = malloc(sizeof(long) * b->nfjob);
+    if (!jobs || !cbeg) {
+        free(jobs); free(cbeg);
+        return awefd__fail(AWEFD_ENOMEM, "final copies");
+    }
+    long c = 0;
+    int j = 0;
+    for (int s = 0; s < b->nshot; s++) {
+        const awefd_cart_args *h = &b->h[s];
+        const long n = b->m[s]->nalloc;
+        if (fp >= 2) {
+            jobs[j] = (awefd_copy_job){h->F[fp], h->F[tp], n};
+            cbeg[j++] = c;
+            c += (n + 1023) / 1024;
+        }
+        if (fc >= 2) {
+            jobs[j] = (awefd_copy_job){h->F[fc], h->F[tc], n};
+            cbeg[j++] = c;
+            c += (n + 1023) / 1024;
+        }
+    }
+    b->fnch = c;
+    hipError_t e = hipMalloc((void **)&b->d_fjobs, sizeof(*jobs) * b->nfjob);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_fcbeg, sizeof(long) * b->nfjob);
+    if (e == hipSuccess) e = hipMemcpy(b->d_fjobs, jobs, sizeof(*jobs) * b->nfjob, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(b->d_fcbeg, cbeg, sizeof(long) * b->nfjob, hipMemcpyHostToDevice);
+    free(jobs);
+    free(cbeg);
+    if (e != hipSuccess) return awefd__fail(AWEFD_EHIP, "final copies: %s", hipGetErrorString(e));
+    return AWEFD_OK;
+}
+
+int awefd_batch_set_depth(awefd_batch *b, int depth)
+{
+    if (!b) return awefd__fail(AWEFD_EINVAL, "NULL batch");
+    if (depth != 1 && depth != 2) return awefd__fail(AWEFD_EINVAL, "batch depth %d (1 or 2)", depth);
+    HIPCHK(hipSetDevice(b->dev));
+    int rc;
+    if (!b->rx && (rc = batch_build(b, 8, 0))) return rc;
+    int sz, sx;
+    if (depth == 2 && !t2_tile(b, &sz, &sx))
+        return awefd__fail(AWEFD_EINVAL, "two steps per pass need tile rx 4 or 8 (tile is %d)", b->rx);
+    b->depth = depth;
+    b->graph_ok = 0;
     return AWEFD_OK;
 }
 
@@ -360,6 +635,75 @@ int awefd_batch_tune(awefd_batch *b, int steps, int *rx_out, int *group_out, dou
     return batch_build(b, RX_CAND[bi], GROUP_CAND[bg]);
 }
 
+/* ms per step of one (depth, tile, group) choice on the batch */
+static int time_choice(awefd_batch *b, int depth, int rx, int group_mb, int steps, hipEvent_t e0, hipEvent_t e1,
+                       double *ms_out)
+{
+    int rc = batch_build(b, rx, group_mb), fp, fc;
+    if (!rc && depth == 2) rc = t2_build(b);
+    if (!rc) rc = batch_reset(b);
+    /* warm the tables */
+    for (int g = 0; g < b->ngroup && !rc && depth == 2; g++) rc = t2_steps(b, g, 1, &fp, &fc);
+    if (!rc && depth == 1) rc = batch_steps(b, 1);
+    if (!rc) rc = (int)hipEventRecord(e0, b->stream);
+    for (int g = 0; g < b->ngroup && !rc && depth == 2; g++) rc = t2_steps(b, g, steps, &fp, &fc);
+    if (!rc && depth == 1) rc = batch_steps(b, steps);
+    if (!rc) rc = (int)hipEventRecord(e1, b->stream);
+    if (!rc) rc = (int)hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (!rc) rc = (int)hipEventElapsedTime(&ms, e0, e1);
+    if (rc > 0) rc = awefd__fail(AWEFD_EHIP, "batch tuning: %s", hipGetErrorString((hipError_t)rc));
+    *ms_out = ms / steps;
+    return rc;
+}
+
+int awefd_batch_tune_depth(awefd_batch *b, int steps, int *rx_out, int *group_out, int *depth_out, double *ms_out)
+{
+    if (!b) return awefd__fail(AWEFD_EINVAL, "NULL batch");
+    HIPCHK(hipSetDevice(b->dev));
+    if (steps < 2) steps = 2;
+    if (steps > b->nt) steps = b->nt;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int bd = 0, bi = -1, bg = -1, rc = AWEFD_OK;
+    double best = 0;
+    memset(b->tune_ms, 0, sizeof b->tune_ms);
+    memset(b->tune2_ms, 0, sizeof b->tune2_ms);
+    for (int d = 1; d <= 2 && !rc; d++)
+        for (int i = 0; i < 4 && !rc; i++) {
+            int sz, sx;
+            b->rx = RX_CAND[i];
+            if (d == 2 && !t2_tile(b, &sz, &sx)) continue;
+            for (int g = 0; g < NGC && !rc; g++) {
+                double ms;
+                if ((rc = time_choice(b, d, RX_CAND[i], GROUP_CAND[g], steps, e0, e1, &ms))) break;
+                (d == 1 ? b->tune_ms : b->tune2_ms)[i][g] = ms;
+                if (bi < 0 || ms < best) {
+                    bd = d;
+                    bi = i;
+                    bg = g;
+                    best = ms;
+                }
+            }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
+    if (rx_out) *rx_out = RX_CAND[bi];
+    if (group_out) *group_out = GROUP_CAND[bg];
+    if (depth_out) *depth_out = bd;
+    if (ms_out)
+        for (int i = 0; i < 4; i++)
+            for (int g = 0; g < NGC; g++) {
+                ms_out[i * NGC + g] = b->tune_ms[i][g];
+                ms_out[4 * NGC + i * NGC + g] = b->tune2_ms[i][g];
+            }
+    if ((rc = batch_build(b, RX_CAND[bi], GROUP_CAND[bg]))) return rc;
+    b->depth = bd;
+    return AWEFD_OK;
+}
+
 /* The time loop of every shot as one graph: field resets, then one batched
  * launch per step of each group. */
 static int batch_capture(awefd_batch *b)
@@ -369,9 +713,38 @@ static int batch_capture(awefd_batch *b)
     if (b->graph) hipGraphDestroy(b->graph);
     b->gexec = NULL;
     b->graph = NULL;
+    int fp = 0, fc = 1, rc;
+    if (b->depth == 2) {
+        if (!b->t2_ok && (rc = t2_build(b))) return rc;
+        /* the final roles are those of any group's schedule */
+        int i = 0, c = 1;
+        const int npair = (b->nt - 1) / 2;
+        for (int k = 0; k < npair; k++) {
+            int d[2], nd = 0;
+            for (int q = 0; q < 4; q++)
+                if (q != i && q != c) d[nd++] = q;
+            i = d[0];
+            c = d[1];
+        }
+        for (int it = 2 * npair; it < b->nt; it++) {
+            int d0 = 0;
+            while (d0 == i || d0 == c) d0++;
+            i = c;
+            c = d0;
+        }
+        if ((rc = t2_final_jobs(b, i, c))) return rc;
+    }
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = batch_reset(b);
-    if (!rc) rc = batch_steps(b, b->nt);
+    rc = batch_reset(b);
+    if (b->depth == 2) {
+        for (int g = 0; g < b->ngroup && !rc; g++) rc = t2_steps(b, g, b->nt, &fp, &fc);
+        if (!rc && b->nfjob) {
+            int e = awefd_launch_copy_jobs(b->d_fjobs, b->d_fcbeg, b->nfjob, b->fnch, b->stream);
+            if (e) rc = awefd__fail(AWEFD_EHIP, "final copies: %s", hipGetErrorString((hipError_t)e));
+        }
+    } else if (!rc) {
+        rc = batch_steps(b, b->nt);
+    }
     hipGraph_t g = NULL;
     hipError_t e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
@@ -399,8 +772,13 @@ int awefd_batch_run(awefd_batch *b)
      * step (u0 after its swap) is buffer u1 for odd nt, u0 for even nt */
     for (int s = 0; s < b->nshot; s++) {
         awefd_model *m = b->m[s];
-        m->dP = b->nt % 2 ? b->u1[s] : b->u0[s];
-        m->dC = b->nt % 2 ? b->u0[s] : b->u1[s];
+        if (b->depth == 2) {
+            m->dP = b->fin_p ? b->u1[s] : b->u0[s];
+            m->dC = b->fin_c ? b->u1[s] : b->u0[s];
+        } else {
+            m->dP = b->nt % 2 ? b->u1[s] : b->u0[s];
+            m->dC = b->nt % 2 ? b->u0[s] : b->u1[s];
+        }
         m->steps_done = b->nt;
     }
     return AWEFD_OK;

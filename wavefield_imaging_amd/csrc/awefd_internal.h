@@ -72,6 +72,13 @@ typedef struct {
     float *seg;
     long nal;
     int K;
+    /* batched two-step pass (2-D): the shot's four field buffers, its tiles
+     * of (64 - 2 nop) z x (rows - 2 nop) x outputs and their injection
+     * buckets (every source node inside a tile's first-step region) */
+    float *F[4];
+    int ntz2, ntx2, nblk2, blk02;
+    const int *ibeg2;
+    const awefd_inj *inj2;
 } awefd_cart_args;
 
 /* Adjoint of the cartesian 2-D loop (adj_kernels.hip). */
@@ -186,6 +193,12 @@ int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void *stream);
 int awefd_launch_fill(float *p, long n, float v, void *stream);
 /* Batched 2-D step over many shots (cart_kernels.hip): shots = device array
  * of per-shot arguments, bshot = shot of every block, rx in {2,4,8,16}. */
+/* Batched 2-D steps it (and it+1 when two) of every shot in one pass:
+ * roles packs the indices into F of the previous and current field and of
+ * the outputs (bits 0-1, 2-3, 4-5, 6-7); extraction blocks write trace rows
+ * it (current field) and it-1 (previous field, it > 0). */
+int awefd_launch_cart_batch_t2(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int two,
+                               int nop, int sse, int rx, void *stream);
 /* recompute step n of every shot inside its checkpoint segment (out of
  * place, no extraction), same tables as awefd_launch_cart_batch2 */
 int awefd_launch_cart_batch_rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n, int nop,

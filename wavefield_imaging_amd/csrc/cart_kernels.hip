@@ -494,6 +494,217 @@ __global__ __launch_bounds__(NW * WAVE) void k_cart_batch2(const awefd_cart_args
     cart_step_body<NOP, 2, SSE, false, RX, NW>(a, prev, cur, it, a.trow, bid, next);
 }
 
+/* ---- Two time steps per pass (batched 2-D shots) ----
+ *
+ * A block of NW waves covers 64 z (one per lane) x ROWS = NW*RX rows of the
+ * first step; its outputs are the inner (64 - 2 NOP) x (ROWS - 2 NOP) nodes,
+ * so neighbouring tiles overlap by 2 NOP in z and x.  The first step is the
+ * single-step body's arithmetic on every lane and row (the z values beyond
+ * the lanes come from the edge lanes' halo loads, the x rows beyond the
+ * block from global loads, the block's own rows through LDS); its result
+ * goes to LDS and is the current field of the second step, whose previous
+ * field is the first step's current field with its pending sponge factor.
+ * Both results are stored for the tile's inner nodes: the pair is the next
+ * pass's input.  Every product, sum and selection is the single-step one, in
+ * the same order, so the fields are those of two single steps bit for bit
+ * (a factor of exactly 1 stands for "no sponge here"). */
+
+/* the 2*NOP z neighbours of v (one per lane), halo values in the edge lanes */
+template <int NOP>
+__device__ __forceinline__ void zneigh(float v, float halo, float (&zm)[NOP], float (&zp)[NOP])
+{
+    float hlo = halo, hhi = halo;
+    zm[0] = shr1(hlo, v);
+    zp[0] = shl1(hhi, v);
+#pragma unroll
+    for (int k = 1; k < NOP; k++) {
+        hlo = shl1(hlo, hlo);
+        hhi = shr1(hhi, hhi);
+        zm[k] = shr1(hlo, zm[k - 1]);
+        zp[k] = shl1(hhi, zp[k - 1]);
+    }
+}
+
+/* One leapfrog update of a node from its current value c, z neighbours,
+ * x neighbours (xm[k-1], xp[k-1] at distance k) and sponged previous value
+ * p, as the single-step body evaluates it (SSE or scalar order). */
+template <int NOP, bool SSE>
+__device__ __forceinline__ float leap2d(const awefd_cart_args &a, float c, const float (&zm)[NOP],
+                                        const float (&zp)[NOP], const float (&xm)[NOP], const float (&xp)[NOP],
+                                        float p, float vv)
+{
+    float lap = c * a.c0;
+    if (SSE) {
+#pragma unroll
+        for (int k = 1; k <= NOP; k++) {
+            lap = lap + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]);
+            lap = lap + a.cx[k - 1] * (xp[k - 1] + xm[k - 1]);
+        }
+        const float t = 2.f * c - p;
+        return t + vv * lap;
+    }
+#pragma unroll
+    for (int k = 1; k <= NOP; k++) {
+        const float t = (zm[k - 1] + zp[k - 1]) * a.cz[k - 1] + (xm[k - 1] + xp[k - 1]) * a.cx[k - 1];
+        lap = lap + t;
+    }
+    const float vl = vv * lap;
+    return (float)(2.0 * (double)c - (double)p + (double)vl);
+}
+
+template <int NOP, bool SSE, int RX, int NW>
+__device__ __forceinline__ void cart_step2_body(const awefd_cart_args &a, int bid, int it, int roles, int two)
+{
+    constexpr int ROWS = NW * RX;
+    constexpr int SZ = WAVE - 2 * NOP;            /* z outputs per tile */
+    constexpr int SX = ROWS - 2 * NOP;            /* x outputs per tile */
+    static_assert(SX > 0, "two-step tile too narrow for the stencil");
+    const float *Pin = a.F[roles & 3], *Cin = a.F[(roles >> 2) & 3];
+    float *D1 = a.F[(roles >> 4) & 3], *D2 = a.F[(roles >> 6) & 3];
+    const int tid = threadIdx.x;
+    if (bid >= a.nblk2) {
+        /* trace rows it (current field) and it-1 (previous field) */
+        const int r = (bid - a.nblk2) * (NW * WAVE) + tid;
+        cart_extract<2>(a, Cin, a.trow + (long)it * a.nr, r);
+        if (it > 0) cart_extract<2>(a, Pin, a.trow + (long)(it - 1) * a.nr, r);
+        return;
+    }
+    const int lane = tid & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    const int lb = xcd_remap(bid, a.nblk2);
+    const int tzi = lb % a.ntz2, txi = lb / a.ntz2;
+    const int n1 = a.n1, n2 = a.n2;
+    const int z0 = tzi * SZ - NOP;                /* z of lane 0           */
+    const int iz = z0 + lane;
+    const bool zin = iz >= 0 && iz < n1;
+    const int bx0 = txi * SX - NOP;               /* the block's first row */
+    const int x0 = bx0 + wv * RX;                 /* wave-uniform          */
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    const unsigned nbytes = (unsigned)a.ld * (unsigned)n2 * 4u;
+    const rsrc_t rC = mk_rsrc(Cin, nbytes), rP = mk_rsrc(Pin, nbytes), rV = mk_rsrc(a.vel, nbytes);
+    const rsrc_t rD1 = mk_rsrc(D1, nbytes), rD2 = mk_rsrc(D2, nbytes);
+    const unsigned vz = zin ? (unsigned)iz * 4u : OOB;
+    int zz = -1;
+    if (lane < NOP) zz = z0 - 1 - lane;
+    else if (lane >= WAVE - NOP) zz = z0 + WAVE + (WAVE - 1 - lane);
+    const unsigned vzh = (zz >= 0 && zz < n1) ? (unsigned)zz * 4u : OOB;
+    const bool zint = iz >= NOP && iz < n1 - NOP;
+    const bool zsp = a.dabc && zin && (iz < a.nb || iz >= n1 - a.nb);
+    const float spz = zsp ? a.sp1[iz] : 1.f;
+    /* this lane's node is an output of the tile */
+    const unsigned vzo = (lane >= NOP && lane < WAVE - NOP) ? vz : OOB;
+
+    float c[RX], p[RX], vv[RX], zh[RX], spx[RX];
+    unsigned xint_m = 0;
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const int ix = x0 + j;
+        const unsigned so = (unsigned)ix * rowb;  /* rows outside the grid: beyond the range */
+        c[j] = bld(rC, vz, so);
+        p[j] = bld<NT>(rP, vz, so);
+        vv[j] = bld<NT>(rV, vz, so);
+        zh[j] = bld(rC, vzh, so);
+        const bool xsp = a.dabc && ix >= 0 && ix < n2 && (ix < a.nb || ix >= n2 - a.nb);
+        spx[j] = xsp ? a.sp2[ix] : 1.f;
+        xint_m |= (ix >= NOP && ix < n2 - NOP) ? 1u << j : 0u;
+    }
+    /* x rows beyond the block, for the edge waves (the others get an
+     * out-of-range offset) */
+    float hg[2 * NOP];
+#pragma unroll
+    for (int h = 0; h < 2 * NOP; h++) {
+        const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
+        const bool outside = hx < bx0 || hx >= bx0 + ROWS;
+        hg[h] = bld(rC, outside ? vz : OOB, (unsigned)hx * rowb);
+    }
+    /* injection entries whose node lies in this tile's first-step region */
+    const int col = tzi + a.ntz2 * txi;
+    const int ib = a.ibeg2[col], ie = a.ibeg2[col + 1];
+
+    __shared__ float xr[2][ROWS][WAVE];
+#pragma unroll
+    for (int j = 0; j < RX; j++) xr[0][wv * RX + j][lane] = c[j];
+    __syncthreads();
+
+    /* first step (it) on every lane and row */
+    float s1[RX];
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        float zm[NOP], zp[NOP], xm[NOP], xp[NOP];
+        zneigh<NOP>(c[j], zh[j], zm, zp);
+#pragma unroll
+        for (int k = 1; k <= NOP; k++) {
+            const int rm = wv * RX + j - k, rp = wv * RX + j + k;   /* block rows */
+            const int hm = j - k < 0 ? NOP + (j - k) : -1;          /* halo slot  */
+            const int hp = j + k >= RX ? NOP + (j + k - RX) : -1;
+            float vm, vp;
+            if (j - k >= 0) vm = c[j - k];
+            else vm = rm >= 0 ? xr[0][rm][lane] : hg[hm];
+            if (j + k < RX) vp = c[j + k];
+            else vp = rp < ROWS ? xr[0][rp][lane] : hg[hp];
+            xm[k - 1] = vm;
+            xp[k - 1] = vp;
+        }
+        float pp = p[j] * spz;
+        pp = pp * spx[j];
+        const float v = leap2d<NOP, SSE>(a, c[j], zm, zp, xm, xp, pp, vv[j]);
+        float nv = (zint && ((xint_m >> j) & 1u)) ? v : pp;
+        for (int e = ib; e < ie; e++) {
+            const awefd_inj en = a.inj2[e];
+            const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
+            const float inj = nv + amp * en.w * vv[j];
+            nv = (zin && en.z == iz && en.x == x0 + j) ? inj : nv;
+        }
+        float w = nv * spz;
+        w = w * spx[j];
+        s1[j] = w;
+        const int r = wv * RX + j;
+        if (r >= NOP && r < ROWS - NOP) bst<NT>(rD1, vzo, (unsigned)(x0 + j) * rowb, w);
+    }
+    if (!two) return;                             /* block-uniform */
+
+#pragma unroll
+    for (int j = 0; j < RX; j++) xr[1][wv * RX + j][lane] = s1[j];
+    __syncthreads();
+
+    /* second step (it + 1): current field s1, previous field c */
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        float zm[NOP], zp[NOP], xm[NOP], xp[NOP];
+        zneigh<NOP>(s1[j], 0.f, zm, zp);          /* edge lanes are not outputs */
+#pragma unroll
+        for (int k = 1; k <= NOP; k++) {
+            const int rm = wv * RX + j - k, rp = wv * RX + j + k;
+            xm[k - 1] = j - k >= 0 ? s1[j - k] : xr[1][max(rm, 0)][lane];
+            xp[k - 1] = j + k < RX ? s1[j + k] : xr[1][min(rp, ROWS - 1)][lane];
+        }
+        float pp = c[j] * spz;
+        pp = pp * spx[j];
+        const float v = leap2d<NOP, SSE>(a, s1[j], zm, zp, xm, xp, pp, vv[j]);
+        float nv = (zint && ((xint_m >> j) & 1u)) ? v : pp;
+        for (int e = ib; e < ie; e++) {
+            const awefd_inj en = a.inj2[e];
+            const float amp = a.wav[(long)(it + 1) * a.wstride + (a.wexpl ? 0 : en.src)];
+            const float inj = nv + amp * en.w * vv[j];
+            nv = (zin && en.z == iz && en.x == x0 + j) ? inj : nv;
+        }
+        float w = nv * spz;
+        w = w * spx[j];
+        const int r = wv * RX + j;
+        if (r >= NOP && r < ROWS - NOP) bst<NT>(rD2, vzo, (unsigned)(x0 + j) * rowb, w);
+    }
+}
+
+template <int NOP, bool SSE, int RX, int NW>
+__global__ __launch_bounds__(NW * WAVE) void k_cart_batch_t2(const awefd_cart_args *__restrict__ shots,
+                                                             const int *__restrict__ bshot, int it, int roles,
+                                                             int two)
+{
+    const int s = __builtin_amdgcn_readfirstlane(bshot[blockIdx.x]);
+    const awefd_cart_args &a = shots[s];
+    cart_step2_body<NOP, SSE, RX, NW>(a, (int)blockIdx.x - a.blk02, it, roles, two);
+}
+
 /* z-high one-way ABC as a separate pass, for grids whose last z tile is too
  * narrow to hold the recurrence (abcone3d_apply, fdutil.c:2110-2118).  One
  * thread per (x, y) column. */
@@ -639,6 +850,27 @@ int launch_batch_sse(const awefd_cart_args *shots, const int *bshot, int nblk, i
     return (int)hipGetLastError();
 }
 
+template <int NOP, bool SSE>
+int launch_t2_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int two, int rx,
+                  hipStream_t s)
+{
+    constexpr int NW = AWEFD_BATCH_NWAVE;
+    switch (rx) {
+    case 4: hipLaunchKernelGGL((k_cart_batch_t2<NOP, SSE, 4, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, two); break;
+    case 8: hipLaunchKernelGGL((k_cart_batch_t2<NOP, SSE, 8, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, two); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+template <int NOP>
+int launch_t2_nop(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int two, int sse,
+                  int rx, hipStream_t s)
+{
+    return sse ? launch_t2_sse<NOP, true>(shots, bshot, nblk, it, roles, two, rx, s)
+               : launch_t2_sse<NOP, false>(shots, bshot, nblk, it, roles, two, rx, s);
+}
+
 template <int NOP, bool RC>
 int launch_batch_nop(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int sse, int rx, hipStream_t s)
 {
@@ -667,6 +899,21 @@ extern "C" int awefd_launch_cart_batch2(const awefd_cart_args *shots, const int 
                                         int nop, int sse, int rx, void *stream)
 {
     return launch_batch<false>(shots, bshot, nblk, it, nop, sse, rx, (hipStream_t)stream);
+}
+
+extern "C" int awefd_launch_cart_batch_t2(const awefd_cart_args *shots, const int *bshot, int nblk, int it,
+                                          int roles, int two, int nop, int sse, int rx, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    switch (nop) {
+    case 1: return launch_t2_nop<1>(shots, bshot, nblk, it, roles, two, sse, rx, s);
+    case 2: return launch_t2_nop<2>(shots, bshot, nblk, it, roles, two, sse, rx, s);
+    case 3: return launch_t2_nop<3>(shots, bshot, nblk, it, roles, two, sse, rx, s);
+    case 4: return launch_t2_nop<4>(shots, bshot, nblk, it, roles, two, sse, rx, s);
+    case 5: return launch_t2_nop<5>(shots, bshot, nblk, it, roles, two, sse, rx, s);
+    case 6: return launch_t2_nop<6>(shots, bshot, nblk, it, roles, two, sse, rx, s);
+    default: return (int)hipErrorInvalidValue;
+    }
 }
 
 extern "C" int awefd_launch_cart_batch_rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n,
