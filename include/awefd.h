@@ -62,6 +62,16 @@ typedef struct awefd_params {
                         (default), 0 = scalar build                           */
     int   jdata;     /* jdata= (receiver decimation; traces keep every step)   */
     int   device;    /* gpu=   HIP device ordinal                             */
+    int   adj;       /* adj=   cartesian: the wavelet is read from its last row
+                        back (MAWEFDcart2Dcpu.c:305-316, MAWEFDcart3Dcpu.c:304-315).
+                        In 2-D the loop also stops after step nt-kt, whose field
+                        is the image (awefd_get_field), extracts no traces, and
+                        without expl injects nr samples per step at the receivers */
+    int   kt;        /* kt=    cartesian 2-D adj: image step nt-kt, 1 <= kt <= nt */
+    int   hybrid;    /* hybridbc= cartesian: with dabc, the hybrid one-way/damped
+                        boundary (apply_abc) replaces the sponge (and the 3-D
+                        one-way faces); nb <= nop widens to 2 nop; a free surface
+                        no longer zeroes (v dt)^2 (MAWEFDcart2Dcpu.c:211-212,276-292) */
 } awefd_params;
 
 typedef struct awefd_model awefd_model;

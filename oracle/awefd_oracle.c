@@ -34,6 +34,10 @@ int orc_threads(void) { return 1; }
 double orc_loop_seconds = 0.0;
 double orc_last_loop_seconds(void) { return orc_loop_seconds; }
 
+/* 1 when the last cartesian 2-D adj=y loop reached its image step nt-kt */
+int orc_imaged = 0;
+int orc_last_imaged(void) { return orc_imaged; }
+
 /* ------------------------------------------------------------------ */
 /* FD coefficients: MAWEFDcart2Dcpu.c:423-551 (3-D: MAWEFDcart3Dcpu.c:404-531) */
 

@@ -53,6 +53,11 @@ typedef struct {
                         0 = scalar order (ts_kernel.c:139-157)                 */
     int   ns, nr;
     int   snap, jsnap;
+    int   adj;       /* cartesian adj=: wavelet read back to front; in 2-D the
+                        loop stops at it = nt-kt (image) and, without expl,
+                        injects nr samples per step at the receivers          */
+    int   kt;        /* cartesian 2-D kt= (image step nt-kt)                   */
+    int   hybrid;    /* cartesian hybridbc=: apply_abc instead of the sponge   */
 } orc_par;
 
 /* Snapshot callback: padded field (the reference's u0 / h_po after the step),
@@ -96,6 +101,7 @@ int orc_spher3d_f64(const orc_par *p, const float *vel,
 
 /* Wall time (s) of the last time loop, and the OpenMP thread count. */
 double orc_last_loop_seconds(void);
+int orc_last_imaged(void);
 int    orc_threads(void);
 
 /* Building blocks exposed for the known-answer tests. */

@@ -24,6 +24,7 @@ class OrcPar(C.Structure):
         ("fdorder", C.c_int), ("optfd", C.c_int), ("dabc", C.c_int),
         ("fsrf", C.c_int), ("expl", C.c_int), ("cden", C.c_int), ("sse", C.c_int),
         ("ns", C.c_int), ("nr", C.c_int), ("snap", C.c_int), ("jsnap", C.c_int),
+        ("adj", C.c_int), ("kt", C.c_int), ("hybrid", C.c_int),
     ]
 
 
@@ -102,11 +103,14 @@ def lint3d(xyz, nzpad, nxpad, nypad, ozpad, oxpad, oypad, dz, dx, dy):
     return jz, jx, jy, w.reshape(na, 8)
 
 
-def padded_shape(kind, n, nb=None, fdorder=4):
-    """Padded grid (fastest axis first) the oracle uses for ``kind``."""
+def padded_shape(kind, n, nb=None, fdorder=4, hybrid=False):
+    """Padded grid (fastest axis first) the oracle uses for ``kind``;
+    hybrid: dabc=y hybridbc=y, which widens nb <= nop to 2 nop."""
     if kind.startswith("cart"):
         nop = fdorder // 2
         b = nop if nb is None or nb < nop else nb
+        if hybrid and b <= nop:
+            b = 2 * nop
     else:
         b = 4 if nb is None or nb < 4 else nb
     dims = 3 if kind.endswith("3d") else 2
@@ -117,7 +121,10 @@ def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, sn
     """Run one oracle propagation.
 
     par keys: n (list), o, d, nt, dt, nb, fdorder, optfd, dabc, fsrf, expl, cden,
-    sse, snap, jsnap.  Returns (traces[nt, nr], final_field or None, snapshots list).
+    sse, snap, jsnap, adj, kt, hybridbc.  Returns (traces[nt, nr], final_field or
+    None, snapshots list); for a cartesian 2-D adj=y run the final field is the
+    one the image is cut from (``imaged()`` tells whether step nt-kt was
+    reached) and the traces stay zero.
     """
     if kind not in KINDS:
         raise ValueError(kind)
@@ -134,14 +141,17 @@ def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, sn
     p.dt = float(par["dt"])
     p.nb = int(par.get("nb", -1) if par.get("nb") is not None else -1)
     p.fdorder = int(par.get("fdorder", 4))
-    p.optfd = int(bool(par.get("optfd", False)))
-    p.dabc = int(bool(par.get("dabc", False)))
-    p.fsrf = int(bool(par.get("free", par.get("fsrf", False))))
-    p.expl = int(bool(par.get("expl", kind.startswith("spher"))))
-    p.cden = int(bool(par.get("cden", True)))
-    p.sse = int(bool(par.get("sse", True)))
-    p.snap = int(bool(par.get("snap", False)))
+    p.optfd = int(_yn(par.get("optfd", False)))
+    p.dabc = int(_yn(par.get("dabc", False)))
+    p.fsrf = int(_yn(par.get("free", par.get("fsrf", False))))
+    p.expl = int(_yn(par.get("expl", kind.startswith("spher"))))
+    p.cden = int(_yn(par.get("cden", True)))
+    p.sse = int(_yn(par.get("sse", True)))
+    p.snap = int(_yn(par.get("snap", False)))
     p.jsnap = int(par.get("jsnap", p.nt))
+    p.adj = int(_yn(par.get("adj", False)))
+    p.kt = int(par.get("kt", 0))
+    p.hybrid = int(_yn(par.get("hybridbc", False)))
     ndim = 3 if kind.endswith("3d") else 2
     sou = np.ascontiguousarray(sou, np.float32).reshape(-1)
     rec = np.ascontiguousarray(rec, np.float32).reshape(-1)
@@ -151,7 +161,7 @@ def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, sn
     wav = np.ascontiguousarray(wav, np.float32).reshape(-1)
     dat = np.zeros((p.nt, p.nr), real)
     if kind.startswith("cart"):
-        shp, _ = padded_shape(kind, n, p.nb, p.fdorder)
+        shp, _ = padded_shape(kind, n, p.nb, p.fdorder, hybrid=bool(p.dabc and p.hybrid))
     else:
         shp, _ = padded_shape(kind, n, p.nb)
     ufin = np.zeros(int(np.prod(shp)), real) if final else None
@@ -178,6 +188,17 @@ def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, sn
     return dat, (ufin.reshape(shp[::-1]) if final else None), shots
 
 
+def imaged():
+    """Whether the last cartesian 2-D adj=y run reached its image step."""
+    return bool(lib().orc_last_imaged())
+
+
+def _yn(v):
+    if isinstance(v, str):
+        return v.lower().startswith("y")
+    return bool(v)
+
+
 def _par2d(par, sou, rec):
     p = OrcPar()
     for i in range(3):
@@ -188,10 +209,10 @@ def _par2d(par, sou, rec):
     p.dt = float(par["dt"])
     p.nb = int(par.get("nb", -1) if par.get("nb") is not None else -1)
     p.fdorder = int(par.get("fdorder", 4))
-    p.optfd = int(bool(par.get("optfd", False)))
-    p.dabc = int(bool(par.get("dabc", False)))
-    p.fsrf = int(bool(par.get("free", par.get("fsrf", False))))
-    p.expl = int(bool(par.get("expl", False)))
+    p.optfd = int(_yn(par.get("optfd", False)))
+    p.dabc = int(_yn(par.get("dabc", False)))
+    p.fsrf = int(_yn(par.get("free", par.get("fsrf", False))))
+    p.expl = int(_yn(par.get("expl", False)))
     p.ns = np.asarray(sou).size // 2
     p.nr = np.asarray(rec).size // 2
     return p

@@ -50,12 +50,23 @@ def _run(prog, args, stdin_path, stdout_path, cwd):
 def test_program_argument_errors(tmp_path):
     w = str(tmp_path / "w.rsf")
     rsf.write(w, np.zeros((10, 1), np.float32), [(0, 1), (0, 1e-8)])
-    r = _run("sfAWEFDcart2D", ["adj=y"], w, str(tmp_path / "o.rsf"), str(tmp_path))
-    assert r.returncode == 1 and b"adj=y" in r.stderr
+    r = _run("sfAWEFDcart2D", ["sinc=y"], w, str(tmp_path / "o.rsf"), str(tmp_path))
+    assert r.returncode == 1 and b"sinc=y" in r.stderr
     r = _run("sfAWEFDcart2D", [], w, str(tmp_path / "o.rsf"), str(tmp_path))
     assert r.returncode == 1 and b"vel=" in r.stderr
     r = _run("sfAWEFDspher3D", ["vel=nothere.rsf"], w, str(tmp_path / "o.rsf"), str(tmp_path))
     assert r.returncode == 1 and b"nothere" in r.stderr
+
+
+def test_adj_needs_an_image_step(tmp_path):
+    """2-D adj=y writes the field at step nt-kt; a kt outside 1..nt (0 when
+    absent) would write no image, so the program stops before the GPU."""
+    case = synth.cart2d(scale=0.05, nt=40)
+    _write_case(tmp_path, "cart2d", case["par"], case["vel"], case["sou"], case["rec"], case["wav"])
+    for extra in ([], ["kt=0"], ["kt=41"]):
+        r = _run("sfAWEFDcart2D", ["vel=vel.rsf", "sou=sou.rsf", "rec=rec.rsf", "adj=y", "expl=y"] + extra,
+                 str(tmp_path / "wav.rsf"), str(tmp_path / "o.rsf"), str(tmp_path))
+        assert r.returncode == 1 and b"kt" in r.stderr, r.stderr
 
 
 # ---------------------------------------------------------------- GPU
@@ -154,3 +165,48 @@ def test_spher3d_program(orc, tmp_path):
     nr, nth, nph = par["n"]
     sl = (slice(5, 5 + nph), slice(5, 5 + nth), slice(5, 5 + nr))
     np.testing.assert_array_equal(wfl, _snaps_window(snaps, sl))
+
+
+@pytest.mark.gpu
+def test_cart2d_program_adj_image(orc, tmp_path):
+    """adj=y kt=: the receiver data injected back to front at the receivers,
+    the image (the interior at step nt-kt) written instead of traces, the
+    snapshots up to that step; hybridbc=y on the same run."""
+    case = synth.cart2d(scale=0.08, nt=150)
+    nt, kt = 150, 20
+    rec = case["rec"][::4]
+    rng = np.random.default_rng(2)
+    data = (case["wav"][:, None] * rng.standard_normal(len(rec))[None, :]).astype(np.float32)
+    par = dict(case["par"], nb=3, dabc=True, hybridbc=True, expl=False, adj=True, kt=kt, snap=True,
+               jsnap=40)
+    _write_case(tmp_path, "cart2d", par, case["vel"], case["sou"], rec, data)
+    args = ["vel=vel.rsf", "sou=sou.rsf", "rec=rec.rsf", "dabc=y", "hybridbc=y", "nb=3", "adj=y",
+            "kt=%d" % kt, "snap=y", "jsnap=40", "wfl=wfl.rsf"]
+    r = _run("sfAWEFDcart2D", args, str(tmp_path / "wav.rsf"), str(tmp_path / "img.rsf"), str(tmp_path))
+    assert r.returncode == 0, r.stderr.decode()
+    img, h = rsf.read(str(tmp_path / "img.rsf"))
+    wfl, hw = rsf.read(str(tmp_path / "wfl.rsf"))
+    _, fin, snaps = orc.run("cart2d", par, case["vel"], case["sou"], rec, data, snaps=True)
+    assert orc.imaged()
+    nb, (nz, nx) = 3, par["n"]
+    assert img.shape == (nx, nz) and int(h["n1"]) == nz and int(h["n2"]) == nx
+    assert float(h["o1"]) == par["o"][0] and float(h["d2"]) == par["d"][1]
+    np.testing.assert_array_equal(img, fin[nb:nb + nx, nb:nb + nz])
+    assert np.abs(img).max() > 0
+    assert [it for it, _ in snaps] == [0, 40, 80, 120] and int(hw["n3"]) == 4
+    np.testing.assert_array_equal(wfl, _snaps_window(snaps, (slice(nb, nb + nx), slice(nb, nb + nz))))
+
+
+@pytest.mark.gpu
+def test_cart3d_program_adj_hybrid(orc, tmp_path):
+    """3-D adj=y reads the wavelet back to front and still writes traces."""
+    case = synth.cart3d(scale=0.06, nt=70)
+    par = dict(case["par"], nb=5, dabc=True, hybridbc=True, free=True, adj=True)
+    _write_case(tmp_path, "cart3d", par, case["vel"], case["sou"], case["rec"], case["wav"])
+    args = ["vel=vel.rsf", "sou=sou.rsf", "rec=rec.rsf", "dabc=y", "hybridbc=y", "free=y", "nb=5", "adj=y"]
+    r = _run("sfAWEFDcart3D", args, str(tmp_path / "wav.rsf"), str(tmp_path / "dat.rsf"), str(tmp_path))
+    assert r.returncode == 0, r.stderr.decode()
+    dat, _ = rsf.read(str(tmp_path / "dat.rsf"))
+    ref, _, _ = orc.run("cart3d", par, case["vel"], case["sou"], case["rec"], case["wav"])
+    assert np.abs(ref).max() > 0
+    np.testing.assert_array_equal(dat, ref)

@@ -1,0 +1,110 @@
+"""hybridbc=y and adj=y of the cartesian programs: HIP kernels vs the CPU
+oracle (bit-exact FP32), through the C ABI."""
+import numpy as np
+import pytest
+
+from wavefield_imaging_amd import awefd
+from test_gpu_cart import _case2d, _case3d, _density
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(kind, par, vel, sou, rec, wav, orc, den=None, jsnap=0):
+    print("case", kind, par["n"], {k: par[k] for k in par if k not in ("n", "o", "d")}, flush=True)
+    with awefd.Model(kind, par) as m:
+        m.set_model(vel, den)
+        m.set_sources(sou)
+        m.set_receivers(rec)
+        m.set_wavelet(wav)
+        tr, sn = m.run(jsnap=jsnap, snapshots=jsnap > 0)
+        field = m.field()
+        # repeat plain runs: the 2-D loop is captured as a graph on the second
+        tr2, _ = m.run()
+        f2 = m.field()
+        tr3, _ = m.run()
+        f3 = m.field()
+    ref, ufin, rsn = orc.run(kind, dict(par, snap=jsnap > 0, jsnap=jsnap or par["nt"]), vel, sou, rec, wav,
+                             den=den, snaps=jsnap > 0)
+    assert np.any(ufin != 0)
+    np.testing.assert_array_equal(tr, ref)
+    np.testing.assert_array_equal(field, ufin)
+    for t, f in ((tr2, f2), (tr3, f3)):
+        np.testing.assert_array_equal(t, ref)
+        np.testing.assert_array_equal(f, ufin)
+    assert [it for it, _ in sn] == [it for it, _ in rsn]
+    for (_, a), (_, b) in zip(sn, rsn):
+        np.testing.assert_array_equal(a, b)
+    return ref, ufin
+
+
+@pytest.mark.parametrize("kw", [
+    dict(), dict(free=True), dict(nb=1), dict(nb=None, fdorder=8), dict(fdorder=6, nb=9),
+    dict(sse=False, free=True), dict(optfd=True), dict(dabc=False, free=True),
+])
+def test_cart2d_hybrid_bit_exact(orc, kw):
+    par, vel, sou, rec, wav = _case2d(nt=160, **dict(dict(hybridbc=True), **kw))
+    sou = np.array([[30.0, 20.0]], np.float32)      # near a corner: the wave crosses every strip
+    _run_both("cart2d", par, vel, sou, rec, wav, orc, jsnap=37)
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(free=True), dict(nb=2), dict(fdorder=6, nb=8), dict(sse=False)])
+def test_cart3d_hybrid_bit_exact(orc, kw):
+    par, vel, sou, rec, wav = _case3d(nt=70, **dict(dict(hybridbc=True), **kw))
+    _run_both("cart3d", par, vel, sou, rec, wav, orc, jsnap=23)
+
+
+def test_hybrid_variable_density(orc):
+    par, vel, sou, rec, wav = _case2d(nt=140, hybridbc=True, cden=False, free=True)
+    _run_both("cart2d", par, vel, sou, rec, wav, orc, den=_density(vel, 3))
+
+
+@pytest.mark.parametrize("expl,kt,kw", [
+    (True, 1, dict()), (True, 45, dict()), (False, 30, dict()), (False, 1, dict(hybridbc=True)),
+    (True, 100, dict(free=True)), (False, 7, dict(fdorder=8, sse=False)),
+])
+def test_cart2d_adj_image_bit_exact(orc, expl, kt, kw):
+    par, vel, sou, rec, wav = _case2d(nt=120, expl=expl, adj=True, kt=kt, **kw)
+    rec = rec[::3]
+    k = 1 if expl else len(rec)
+    rng = np.random.default_rng(kt)
+    wav = (wav[:, None] * rng.standard_normal(k)[None, :]).astype(np.float32)
+    ref, img = _run_both("cart2d", par, vel, sou, rec, wav, orc, jsnap=11)
+    assert orc.imaged() and np.all(ref == 0)
+
+
+@pytest.mark.parametrize("expl,kw", [(True, dict()), (False, dict(hybridbc=True)), (False, dict(free=True))])
+def test_cart3d_adj_bit_exact(orc, expl, kw):
+    par, vel, sou, rec, wav = _case3d(nt=60, expl=expl, adj=True, **kw)
+    sou = np.concatenate([sou, sou + np.float32([[20.0, -10.0, 5.0]])])
+    w = np.stack([wav, -0.7 * wav[::-1]], 1).astype(np.float32)
+    _run_both("cart3d", par, vel, sou, rec, wav if expl else w, orc)
+
+
+def test_adj_and_hybrid_errors(orc):
+    par, vel, sou, rec, wav = _case2d(nt=50, adj=True, kt=0)
+    with pytest.raises(awefd.L.AwefdError):
+        awefd.Model("cart2d", par)
+    with pytest.raises(awefd.L.AwefdError):
+        awefd.Model("cart2d", dict(par, kt=51))
+    # without expl the 2-D adjoint loop reads nr samples per step
+    with awefd.Model("cart2d", dict(par, kt=5)) as m:
+        m.set_model(vel)
+        m.set_sources(sou)
+        m.set_receivers(rec)
+        m.set_wavelet(wav)                      # nt samples, not nt * nr
+        with pytest.raises(awefd.L.AwefdError):
+            m.run()
+    # batches take neither mode
+    ms = []
+    for kw in (dict(adj=True, kt=3, expl=True), dict(hybridbc=True)):
+        p = _case2d(nt=50, **kw)[0]
+        m = awefd.Model("cart2d", p)
+        m.set_model(vel)
+        m.set_sources(sou)
+        m.set_receivers(rec)
+        m.set_wavelet(wav)
+        ms.append(m)
+        with pytest.raises(awefd.L.AwefdError):
+            awefd.Batch([m])
+    for m in ms:
+        m.close()

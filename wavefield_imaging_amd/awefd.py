@@ -29,7 +29,7 @@ def make_params(kind, par):
     """Build an awefd_params from program parameters.
 
     par: n (list), o, d, nt, dt and any getpar key (nb, fdorder, optfd, dabc,
-    free/fsrf, expl, cden, sse, jdata, gpu).
+    free/fsrf, expl, cden, sse, jdata, gpu, adj, kt, hybridbc).
     """
     k = L.KIND_NAMES[kind] if isinstance(kind, str) else int(kind)
     lib = L.load()
@@ -57,6 +57,12 @@ def make_params(kind, par):
         p.sse = int(_yn(par["sse"]))
     if "jdata" in par:
         p.jdata = int(par["jdata"])
+    if "adj" in par:
+        p.adj = int(_yn(par["adj"]))
+    if "kt" in par:
+        p.kt = int(par["kt"])
+    if "hybridbc" in par:
+        p.hybrid = int(_yn(par["hybridbc"]))
     p.device = int(par.get("gpu", 0))
     return p
 

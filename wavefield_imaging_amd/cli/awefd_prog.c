@@ -110,6 +110,9 @@ static void window_axis(float oq, float dq, float opad, float d, int *f0, int *j
     *jq = (int)floor(dq / d);
 }
 
+static void write_traces(const float *tr, int nt, int nr, int jdata, int cart, const rsf_t *rec, float ot,
+                         float dt);
+
 int main(int argc, char **argv)
 {
     const int kind = AWEFD_PROG_KIND;
@@ -123,7 +126,7 @@ int main(int argc, char **argv)
     awefd_params p;
     chk(awefd_params_default(kind, &p), "defaults");
     int verb = 0, snap, bnds, expl, dabc = 0, cden = 0, adj = 0, fsrf = 0, optfd = 0, fdorder = 4;
-    int hybrid = 0, sinc = 0, gpu = 0, jdata = 1, jsnap, nb = -1;
+    int hybrid = 0, sinc = 0, gpu = 0, jdata = 1, jsnap, nb = -1, kt = 0;
     par_bool("verb", &verb);
     if (cart) {
         /* MAWEFDcart2Dcpu.c:116-128, MAWEFDcart3Dcpu.c:116-128 */
@@ -139,8 +142,7 @@ int main(int argc, char **argv)
         par_int("fdorder", &fdorder);
         par_bool("hybridbc", &hybrid);
         par_bool("sinc", &sinc);
-        if (adj) die("adj=y (reverse-time injection) is not available in this build");
-        if (hybrid) die("hybridbc=y is not available in this build");
+        par_int("kt", &kt);   /* MAWEFDcart2Dcpu.c:165-167: 0 when absent */
         if (sinc) die("sinc=y is not available in this build");
     } else {
         /* MAWEFDspher2Dcpu.c:56-60 (snap, bnds default y), MAWEFDspher3Dcpu.c:57-61 */
@@ -189,6 +191,11 @@ int main(int argc, char **argv)
     p.cden = cden;
     p.jdata = jdata;
     p.device = gpu;
+    p.adj = cart && adj;
+    p.kt = kt;
+    p.hybrid = cart && hybrid;
+    /* 2-D adj=y: the output is the image at step nt-kt, not traces */
+    const int image = kind == AWEFD_CART2D && adj;
 
     /* point counts: axis 2 of sou=/rec= (the spherical 2-D program also
      * multiplies by axis 3, MAWEFDspher2Dcpu.c:90-91) */
@@ -215,8 +222,16 @@ int main(int argc, char **argv)
     memset(&sn, 0, sizeof sn);
     sn.kind = kind;
     sn.bnds = bnds;
-    rsf_t wh;
+    rsf_t wh, win;
     rsf_axes_init(&wh);
+    rsf_axes_init(&win);
+    for (int a = 0; a < 3; a++) {
+        /* the image axes: the model's, or the snapshot window's when snap=y
+         * (MAWEFDcart2Dcpu.c:161-164,186-196: acz/acx) */
+        win.n[a] = p.n[a];
+        win.o[a] = p.o[a];
+        win.d[a] = p.d[a];
+    }
     if (snap) {
         const char *wp;
         if (!par_string("wfl", &wp)) die("snap=y needs wfl=");
@@ -225,7 +240,9 @@ int main(int argc, char **argv)
             sn.npad[a] = dims[a];
             opad[a] = a < ndim ? p.o[a] - (float)nbe * p.d[a] : 0.f;
         }
-        const int ntsnap = (nt - 1) / jsnap + 1;
+        /* the 2-D adj=y loop stops after step nt-kt: the header counts the
+         * frames written (the reference keeps (nt-1)/jsnap+1 and writes fewer) */
+        const int ntsnap = image ? (nt - kt) / jsnap + 1 : (nt - 1) / jsnap + 1;
         if (cart) {
             /* snapshot window: nqz/nqx/nqy, oqz..., dqz... (MAWEFDcart2Dcpu.c:186-207,
              * MAWEFDcart3Dcpu.c:178-205) */
@@ -241,6 +258,9 @@ int main(int argc, char **argv)
                 wh.n[a] = nq;
                 wh.o[a] = oq;
                 wh.d[a] = dq;
+                win.n[a] = nq;
+                win.o[a] = oq;
+                win.d[a] = dq;
                 if (a >= ndim) {
                     sn.nq[a] = 1;
                     sn.jq[a] = 1;
@@ -322,16 +342,55 @@ int main(int argc, char **argv)
         if (!sn.fp) die("%s", rsf_error());
     }
 
+    if (image && (win.n[0] != p.n[0] || win.n[1] != p.n[1]))
+        die("adj=y with a snapshot window of %d x %d nodes: the image is the whole %d x %d model "
+            "(the reference would write it into the smaller window)", win.n[0], win.n[1], p.n[0], p.n[1]);
     float *tr = malloc(sizeof(float) * (size_t)nt * (nr > 0 ? nr : 1));
     if (!tr) die("out of memory");
     chk(awefd_run(m, tr, snap ? jsnap : 0, snap ? snap_cb : NULL, &sn), "run");
 
-    /* traces: n1 = receivers (axis 2 of rec=), n2 = time */
+    if (image) {
+        /* the interior of u0 at step nt-kt (MAWEFDcart2Dcpu.c:353-370) */
+        float *u = malloc(sizeof(float) * (size_t)dims[0] * dims[1]);
+        float *img = malloc(sizeof(float) * (size_t)p.n[0] * p.n[1]);
+        if (!u || !img) die("out of memory");
+        chk(awefd_get_field(m, u), "field");
+        for (int ix = 0; ix < p.n[1]; ix++)
+            memcpy(img + (size_t)ix * p.n[0], u + (size_t)(nbe + ix) * dims[0] + nbe, sizeof(float) * p.n[0]);
+        rsf_t out;
+        rsf_axes_init(&out);
+        for (int a = 0; a < 2; a++) {
+            out.n[a] = p.n[a];
+            out.o[a] = win.o[a];
+            out.d[a] = win.d[a];
+        }
+        out.ndim = 2;
+        FILE *fo = rsf_open_out(NULL, &out, par_history());
+        if (!fo) die("%s", rsf_error());
+        if (rsf_append(fo, img, (long)p.n[0] * p.n[1]) || rsf_close(fo)) die("%s", rsf_error());
+        free(u);
+        free(img);
+    } else {
+        write_traces(tr, nt, nr, jdata, cart, &rec, ot, dt);
+    }
+    if (snap && rsf_close(sn.fp)) die("%s", rsf_error());
+    if (verb) fprintf(stderr, "\nElapsed time: %lf s\n", now() - t0);
+
+    awefd_destroy(m);
+    free(tr);
+    free(sn.buf);
+    rsf_free(&wav); rsf_free(&vel); rsf_free(&sou); rsf_free(&rec); rsf_free(&den);
+    return 0;
+}
+
+/* traces: n1 = receivers (axis 2 of rec=), n2 = time */
+static void write_traces(const float *tr, int nt, int nr, int jdata, int cart, const rsf_t *rec, float ot, float dt)
+{
     rsf_t out;
     rsf_axes_init(&out);
     out.n[0] = nr;
-    out.o[0] = rec.o[1];
-    out.d[0] = rec.d[1];
+    out.o[0] = rec->o[1];
+    out.d[0] = rec->d[1];
     snprintf(out.label[0], sizeof out.label[0], "r");
     out.n[1] = cart ? (nt - 1) / jdata + 1 : nt / jdata;
     out.o[1] = ot;
@@ -343,12 +402,4 @@ int main(int argc, char **argv)
     for (int k = 0; k < out.n[1]; k++)
         if (rsf_append(fo, tr + (long)k * jdata * nr, nr)) die("%s", rsf_error());
     if (rsf_close(fo)) die("%s", rsf_error());
-    if (snap && rsf_close(sn.fp)) die("%s", rsf_error());
-    if (verb) fprintf(stderr, "\nElapsed time: %lf s\n", now() - t0);
-
-    awefd_destroy(m);
-    free(tr);
-    free(sn.buf);
-    rsf_free(&wav); rsf_free(&vel); rsf_free(&sou); rsf_free(&rec); rsf_free(&den);
-    return 0;
 }

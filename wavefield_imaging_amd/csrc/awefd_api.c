@@ -84,7 +84,8 @@ void awefd_destroy(awefd_model *m)
                      (void **)&m->d_rw,   (void **)&m->d_inj,   (void **)&m->d_ibeg,
                      (void **)&m->d_wav,  (void **)&m->d_traces, (void **)&m->d_sj,
                      (void **)&m->d_sw,   (void **)&m->d_srefl, (void **)&m->d_damp,
-                     (void **)&m->d_metric, (void **)&m->d_abcrules};
+                     (void **)&m->d_metric, (void **)&m->d_abcrules, (void **)&m->d_damp_h,
+                     (void **)&m->d_wavr};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free_dev(ptrs[i]);
     if (m->ev) {
         for (int i = 0; i < m->nev; i++) hipEventDestroy(m->ev[i]);
@@ -94,6 +95,7 @@ void awefd_destroy(awefd_model *m)
     free(m->h_sj); free(m->h_sw); free(m->h_rj); free(m->h_rw);
     free(m->h_sp[0]); free(m->h_sp[1]); free(m->h_sp[2]);
     free(m->h_srefl);
+    free(m->h_wav);
     awefd__spher_free(m);
     awefd__adjoint_free(m);
     if (m->cgexec) hipGraphExecDestroy(m->cgexec);
@@ -125,7 +127,7 @@ static void cart_tiling(awefd_model *m)
     m->ntz = (n1 + 63) / 64;
     m->tz = 64;
     /* the z-high ABC recurrence needs its nop+2 nodes in the last tile */
-    m->zsep = m->p.dabc && m->ndim == 3 && m->ntz > 1 && n1 - 64 * (m->ntz - 1) < m->nop + 2;
+    m->zsep = m->sponge && m->ndim == 3 && m->ntz > 1 && n1 - 64 * (m->ntz - 1) < m->nop + 2;
     m->ntx = (n2 + AWEFD_CART_TX - 1) / AWEFD_CART_TX;
     if (m->ndim == 3) {
         int cols = m->ntz * m->ntx;
@@ -181,11 +183,29 @@ int awefd_create(const awefd_params *p, awefd_model **out)
             free(m);
             return awefd__fail(AWEFD_EINVAL, "fdorder=%d unsupported (even, 2..12)", fdorder);
         }
-        /* MAWEFDcart2Dcpu.c:211: nb < nop or absent -> nop */
+        /* MAWEFDcart2Dcpu.c:211-212: nb < nop or absent -> nop; the hybrid
+         * boundary widens nb <= nop to 2 nop */
         m->nb = p->nb < m->nop ? m->nop : p->nb;
+        m->hyb = p->dabc && p->hybrid;
+        m->sponge = p->dabc && !p->hybrid;
+        if (m->hyb && m->nb <= m->nop) m->nb = 2 * m->nop;
+        m->adj2 = p->adj && m->ndim == 2;
+        m->inj_rec = m->adj2 && !p->expl;
+        m->nsteps = p->nt;
+        if (m->adj2) {
+            /* the loop images at it = nt-kt and stops (MAWEFDcart2Dcpu.c:353-370);
+             * outside 1..nt it never does and writes nothing */
+            if (p->kt < 1 || p->kt > p->nt) {
+                free(m);
+                return awefd__fail(AWEFD_EINVAL, "adj=y needs 1 <= kt <= nt (kt=%d, nt=%d): the image is "
+                                   "the field at step nt-kt", p->kt, p->nt);
+            }
+            m->nsteps = p->nt - p->kt + 1;
+        }
     } else {
         m->nop = 4; /* NOP of spher_utils.c:326 */
         m->nb = p->nb < 4 ? 4 : p->nb;
+        m->nsteps = p->nt;
     }
     m->ncell = 1;
     for (int a = 0; a < 3; a++) {
@@ -227,14 +247,11 @@ int awefd_create(const awefd_params *p, awefd_model **out)
                                    "buffer window of the step kernel", m->np[0], m->np[1]);
             }
         }
-        if (m->ndim == 3 && p->dabc && (m->np[0] < 2 * m->nop + 4 || m->np[1] < 2 * m->nop + 4
+        if (m->ndim == 3 && m->sponge && (m->np[0] < 2 * m->nop + 4 || m->np[1] < 2 * m->nop + 4
                                         || m->np[2] < 2 * m->nop + 4)) {
             awefd_destroy(m);
             return awefd__fail(AWEFD_EINVAL, "3-D one-way ABC needs >= %d padded nodes per axis",
                                2 * m->nop + 4);
-        }
-        if (p->cden == 0 && 0) {
-            /* variable density handled when a density is supplied */
         }
     }
     const size_t bytes = sizeof(float) * (size_t)m->nalloc;
@@ -328,14 +345,21 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
     awefd_pad(vel, v, m->p.n, nbs);
     const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2];
     int rc = AWEFD_OK;
-    if (m->p.dabc && m->ndim == 3) {
-        /* abcone3d_make(nb, dt, vel, ...) on the raw velocity, MAWEFDcart3Dcpu.c:270 */
+    if ((m->sponge && m->ndim == 3) || m->hyb) {
+        /* abcone2d_make/abcone3d_make(nb, dt, vel, ...) on the raw velocity,
+         * MAWEFDcart2Dcpu.c:276, MAWEFDcart3Dcpu.c:270; the 2-D tables are
+         * bz*[n2] and bx*[n1] (the 3-D layout with n3 = 1) */
         const long sx = n1, sy = (long)n1 * n2;
         const int nb = m->nb;
         float *t[6];
         size_t sz[6] = {(size_t)n2 * n3, (size_t)n2 * n3, (size_t)n1 * n3,
                         (size_t)n1 * n3, (size_t)n1 * n2, (size_t)n1 * n2};
         for (int i = 0; i < 6; i++) t[i] = malloc(sizeof(float) * sz[i]);
+        if (!t[0] || !t[1] || !t[2] || !t[3] || !t[4] || !t[5]) {
+            for (int i = 0; i < 6; i++) free(t[i]);
+            free(v);
+            return awefd__fail(AWEFD_ENOMEM, "one-way tables");
+        }
         for (int iy = 0; iy < n3; iy++)
             for (int ix = 0; ix < n2; ix++) {
                 t[0][iy * n2 + ix] = awefd_oneway_cart(v[iy * sy + ix * sx + nb], m->p.dt, m->p.d[0]);
@@ -346,21 +370,28 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
                 t[2][iy * n1 + iz] = awefd_oneway_cart(v[iy * sy + nb * sx + iz], m->p.dt, m->p.d[1]);
                 t[3][iy * n1 + iz] = awefd_oneway_cart(v[iy * sy + (n2 - nb - 1) * sx + iz], m->p.dt, m->p.d[1]);
             }
-        for (int ix = 0; ix < n2; ix++)
+        for (int ix = 0; ix < n2 && m->ndim == 3; ix++)
             for (int iz = 0; iz < n1; iz++) {
                 t[4][ix * n1 + iz] = awefd_oneway_cart(v[(long)nb * sy + ix * sx + iz], m->p.dt, m->p.d[2]);
                 t[5][ix * n1 + iz] = awefd_oneway_cart(v[(long)(n3 - nb - 1) * sy + ix * sx + iz], m->p.dt, m->p.d[2]);
             }
-        for (int i = 0; i < 6 && rc == 0; i++)
+        for (int i = 0; i < (m->ndim == 3 ? 6 : 4) && rc == 0; i++)
             rc = upload(m, (void **)&m->dtab[i], t[i], sizeof(float) * sz[i]);
         for (int i = 0; i < 6; i++) free(t[i]);
     }
-    if (m->p.dabc && rc == 0) {
+    if (m->p.dabc && m->nb > 4096) {
+        free(v);
+        return awefd__fail(AWEFD_EINVAL, "nb=%d too large", m->nb);
+    }
+    if (m->hyb && rc == 0) {
+        /* damp_make(nb - nop), MAWEFDcart2Dcpu.c:407-421: the sponge_make
+         * table over nb - nop nodes */
         float w[4096];
-        if (m->nb > 4096) {
-            free(v);
-            return awefd__fail(AWEFD_EINVAL, "nb=%d too large", m->nb);
-        }
+        awefd_sponge_weights(m->nb - m->nop, w);
+        rc = upload(m, (void **)&m->d_damp_h, w, sizeof(float) * (m->nb - m->nop));
+    }
+    if (m->sponge && rc == 0) {
+        float w[4096];
         awefd_sponge_weights(m->nb, w);
         for (int a = 0; a < 3 && rc == 0; a++) {
             free(m->h_sp[a]);
@@ -370,10 +401,11 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
             rc = upload(m, (void **)&m->dsp[a], m->h_sp[a], sizeof(float) * m->np[a]);
         }
     }
-    /* vel <- vel * (vel * dt^2) ; free surface zeroes z in [0, nb] */
+    /* vel <- vel * (vel * dt^2) ; free surface zeroes z in [0, nb], except
+     * under hybridbc (MAWEFDcart2Dcpu.c:289-295) */
     const float dt2 = m->p.dt * m->p.dt;
     for (long i = 0; i < m->ncell; i++) v[i] *= v[i] * dt2;
-    if (m->p.fsrf)
+    if (m->p.fsrf && !m->p.hybrid)
         for (long c = 0; c < (long)n2 * n3; c++)
             for (int iz = 0; iz <= m->nb && iz < n1; iz++) v[c * n1 + iz] = 0.f;
     if (rc == 0) {
@@ -441,13 +473,17 @@ static int set_points(awefd_model *m, int which, int n, const float *coords)
 }
 
 /* Injection buckets of the cartesian step kernel (one per z-tile/x-tile
- * column of a tile tz x tx), holding every (node, source, weight)
- * contribution; host arrays, freed by the caller. */
+ * column of a tile tz x tx), holding every (node, point, weight)
+ * contribution of the injected points (the sources, or the receivers in a
+ * 2-D adj=y loop without expl); host arrays, freed by the caller. */
 int awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj **inj, int **ibeg_out,
                         int *nbucket_out, long *ninj)
 {
     const int nc = m->ndim == 3 ? 8 : 4;
-    const long n = (long)m->ns * nc;
+    const int npt = m->inj_rec ? m->nr : m->ns;
+    const int *pj = m->inj_rec ? m->h_rj : m->h_sj;
+    const float *pw = m->inj_rec ? m->h_rw : m->h_sw;
+    const long n = (long)npt * nc;
     const int ntx = (m->np[1] + tx - 1) / tx;
     const int nbucket = ntz * ntx;
     awefd_inj_key *keys = malloc(sizeof(*keys) * (n > 0 ? n : 1));
@@ -457,14 +493,14 @@ int awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj
         free(keys); free(e); free(ibeg);
         return awefd__fail(AWEFD_ENOMEM, "injection buckets");
     }
-    for (int s = 0; s < m->ns; s++)
+    for (int s = 0; s < npt; s++)
         for (int q = 0; q < nc; q++) {
             awefd_inj_key *k = &keys[(long)s * nc + q];
-            k->e.z = m->h_sj[3 * s] + (q & 1);
-            k->e.x = m->h_sj[3 * s + 1] + ((q >> 1) & 1);
-            k->e.y = m->ndim == 3 ? m->h_sj[3 * s + 2] + ((q >> 2) & 1) : 0;
+            k->e.z = pj[3 * s] + (q & 1);
+            k->e.x = pj[3 * s + 1] + ((q >> 1) & 1);
+            k->e.y = m->ndim == 3 ? pj[3 * s + 2] + ((q >> 2) & 1) : 0;
             k->e.src = s;
-            k->e.w = m->h_sw[8 * s + q];
+            k->e.w = pw[8 * s + q];
             k->bucket = (k->e.z / tz) + ntz * (k->e.x / tx);
         }
     awefd_sort_inj(keys, n, nbucket, e, ibeg);
@@ -495,7 +531,8 @@ int awefd_set_sources(awefd_model *m, int ns, const float *coords)
     int rc = set_points(m, 0, ns, coords);
     if (rc) return rc;
     HIPCHK(hipSetDevice(m->dev));
-    rc = is_cart(m) ? cart_build_inj(m) : awefd__spher_set_sources(m);
+    if (!is_cart(m)) rc = awefd__spher_set_sources(m);
+    else if (!m->inj_rec) rc = cart_build_inj(m);
     if (!rc) m->src_set = 1;
     m->gen++;
     return rc;
@@ -508,6 +545,7 @@ int awefd_set_receivers(awefd_model *m, int nr, const float *coords)
     HIPCHK(hipSetDevice(m->dev));
     rc = upload(m, (void **)&m->d_rj, m->h_rj, sizeof(int) * 3 * (nr > 0 ? nr : 1));
     if (!rc) rc = upload(m, (void **)&m->d_rw, m->h_rw, sizeof(float) * 8 * (nr > 0 ? nr : 1));
+    if (!rc && m->inj_rec) rc = cart_build_inj(m);
     if (!rc) m->rec_set = 1;
     m->gen++;
     return rc;
@@ -518,6 +556,14 @@ int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav)
     if (!m || nsamples < 1 || !wav) return awefd__fail(AWEFD_EINVAL, "awefd_set_wavelet: bad argument");
     HIPCHK(hipSetDevice(m->dev));
     int rc = upload(m, (void **)&m->d_wav, wav, sizeof(float) * nsamples);
+    if (!rc && is_cart(m) && m->p.adj) {
+        /* adj=y reads the rows back to front; the row length depends on the
+         * points set later, so the reversed copy is made at run time */
+        free(m->h_wav);
+        m->h_wav = malloc(sizeof(float) * nsamples);
+        if (!m->h_wav) rc = awefd__fail(AWEFD_ENOMEM, "wavelet copy");
+        else memcpy(m->h_wav, wav, sizeof(float) * nsamples);
+    }
     if (!rc) {
         m->nwav = nsamples;
         m->wav_set = 1;
@@ -614,7 +660,7 @@ static hipError_t copy_field_out(const awefd_model *m, float *host, const float 
  * reference's u0 view), in the reference's z, x, y order. */
 static void cart_apply_pending(const awefd_model *m, float *u)
 {
-    if (!m->p.dabc) return;
+    if (!m->sponge) return;
     const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2], nb = m->nb;
     for (int iy = 0; iy < n3; iy++)
         for (int ix = 0; ix < n2; ix++)
@@ -667,12 +713,12 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
     a->ychunk = m->ychunk;
     a->nty = m->nty;
     a->nblk = m->nblk;
-    a->dabc = m->p.dabc;
+    a->dabc = m->sponge;
     a->nb = m->nb;
     a->sp1 = m->dsp[0];
     a->sp2 = m->dsp[1];
     a->sp3 = m->dsp[2];
-    a->abc = m->p.dabc && m->ndim == 3;
+    a->abc = m->sponge && m->ndim == 3;
     a->fsrf = m->p.fsrf;
     a->bzl = m->dtab[0];
     a->bzh = m->dtab[1];
@@ -680,12 +726,13 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
     a->bxh = m->dtab[3];
     a->byl = m->dtab[4];
     a->byh = m->dtab[5];
+    a->damp = m->d_damp_h;
     a->ibeg = m->d_ibeg;
     a->inj = m->d_inj;
-    a->wav = m->d_wav;
+    a->wav = m->p.adj ? m->d_wavr : m->d_wav;
     a->wexpl = m->p.expl;
-    a->wstride = m->p.expl ? 1 : m->ns;
-    a->nr = m->nr;
+    a->wstride = m->p.expl ? 1 : m->inj_rec ? m->nr : m->ns;
+    a->nr = m->adj2 ? 0 : m->nr;     /* the 2-D adj=y loop extracts no traces */
     a->rj = m->d_rj;
     a->rw = m->d_rw;
 }
@@ -697,10 +744,25 @@ static int check_ready(awefd_model *m)
     if (!m->rec_set) return awefd__fail(AWEFD_ESTATE, "receivers not set");
     if (!m->wav_set) return awefd__fail(AWEFD_ESTATE, "wavelet not set");
     long need;
-    if (is_cart(m)) need = (long)m->p.nt * (m->p.expl ? 1 : m->ns);
+    const long row = m->p.expl ? 1 : m->inj_rec ? m->nr : m->ns;
+    if (is_cart(m)) need = (long)m->p.nt * row;
     else need = m->p.nt;
     if (m->nwav < need)
         return awefd__fail(AWEFD_EINVAL, "wavelet has %ld samples, the loop reads %ld", m->nwav, need);
+    if (is_cart(m) && m->p.adj && (!m->wrev_ok || m->wrev_gen != m->gen)) {
+        /* adj=y: step it reads row nt-1-it (MAWEFDcart2Dcpu.c:305-316,
+         * MAWEFDcart3Dcpu.c:304-315); the kernels read row it of this copy */
+        float *r = malloc(sizeof(float) * (size_t)(need > 0 ? need : 1));
+        if (!r) return awefd__fail(AWEFD_ENOMEM, "reversed wavelet");
+        for (int it = 0; it < m->p.nt; it++)
+            memcpy(r + (long)it * row, m->h_wav + (long)(m->p.nt - 1 - it) * row, sizeof(float) * row);
+        free_dev((void **)&m->d_wavr);
+        int rc = upload(m, (void **)&m->d_wavr, r, sizeof(float) * (size_t)(need > 0 ? need : 1));
+        free(r);
+        if (rc) return rc;
+        m->wrev_gen = m->gen;
+        m->wrev_ok = 1;
+    }
     return AWEFD_OK;
 }
 
@@ -719,7 +781,7 @@ static int cart_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, vo
 {
     awefd_cart_args a;
     awefd__cart_args(m, &a);
-    const int nt = m->p.nt;
+    const int nt = m->nsteps;
     float *P = m->dP, *C = m->dC;
     float *host = NULL;
     int npairs = 0, rc = AWEFD_OK;
@@ -734,6 +796,7 @@ static int cart_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, vo
         if (m->prof) HIPCHK(hipEventRecord(m->ev[2 * npairs + 1], m->stream));
         if (m->prof) npairs++;
         if (!e && a.abc) e = awefd_launch_cart_abc_xy(&a, m->nop, m->stream);
+        if (!e && m->hyb) e = awefd_launch_cart_hybrid(&a, m->nop, m->ndim, m->stream);
         if (e) return awefd__fail(AWEFD_EHIP, "step %d launch: %s", it, hipGetErrorString((hipError_t)e));
         float *t = P;
         P = C;
@@ -774,12 +837,13 @@ static int cart_loop_graph(awefd_model *m, float *dtr)
         float *P = m->dP, *C = m->dC;
         HIPCHK(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
         int e = 0;
-        for (int it = 0; it < m->p.nt && !e; it++) {
+        for (int it = 0; it < m->nsteps && !e; it++) {
             a.P = P;
             a.C = C;
             a.it = it;
             a.trow = trow0 + (long)it * m->nr;
             e = awefd_launch_cart_step(&a, m->nop, m->ndim, m->p.sse, a.rho != NULL, m->stream);
+            if (!e && m->hyb) e = awefd_launch_cart_hybrid(&a, m->nop, m->ndim, m->stream);
             float *t = P;
             P = C;
             C = t;
@@ -802,7 +866,7 @@ static int cart_loop_graph(awefd_model *m, float *dtr)
     HIPCHK(hipGraphLaunch(m->cgexec, m->stream));
     m->dP = m->cPf;
     m->dC = m->cCf;
-    m->steps_done = m->p.nt;
+    m->steps_done = m->nsteps;
     return AWEFD_OK;
 }
 
@@ -837,7 +901,7 @@ int awefd_run(awefd_model *m, float *traces, int jsnap, awefd_snap_fn cb, void *
     if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
     int rc = run_common(m, NULL, jsnap, cb, user);
     if (rc) return rc;
-    if (traces && m->nr > 0)
+    if (traces && m->nr > 0 && !m->adj2)
         HIPCHK(hipMemcpy(traces, m->d_traces, sizeof(float) * (size_t)m->p.nt * m->nr,
                          hipMemcpyDeviceToHost));
     return AWEFD_OK;

@@ -168,6 +168,8 @@ int awefd_batch_create(int nshot, awefd_model *const *models, awefd_batch **out)
             return awefd__fail(AWEFD_EINVAL, "batch: shot %d differs from shot 0 in nt, fdorder, sse or device", s);
         if (m->drho && !m->p.cden)
             return awefd__fail(AWEFD_EINVAL, "batch: shot %d has variable density (not batched)", s);
+        if (m->p.adj || m->hyb)
+            return awefd__fail(AWEFD_EINVAL, "batch: shot %d uses adj=y or hybridbc=y (not batched)", s);
         if (!m->model_set || !m->src_set || !m->rec_set || !m->wav_set)
             return awefd__fail(AWEFD_ESTATE, "batch: shot %d is not fully set up", s);
         const long need = (long)m->p.nt * (m->p.expl ? 1 : m->ns);

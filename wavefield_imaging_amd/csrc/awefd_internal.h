@@ -49,9 +49,11 @@ typedef struct {
     /* sponge: per-axis factors, 1 outside the strips */
     int dabc, nb;
     const float *sp1, *sp2, *sp3;
-    /* 3-D one-way ABC */
+    /* 3-D one-way ABC; the same tables (2-D: bz*, bx*) for the hybrid
+     * boundary, whose transition strip blends by damp[nb - nop] */
     int abc, fsrf;
     const float *bzl, *bzh, *bxl, *bxh, *byl, *byh;
+    const float *damp;
     /* injection, bucketed by (z tile, x tile) column, sorted by (y, z, x, src) */
     const int *ibeg;
     const awefd_inj *inj;
@@ -190,6 +192,9 @@ typedef struct {
 int awefd_launch_cart_step(const awefd_cart_args *a, int nop, int ndim, int sse, int varden,
                            void *stream);
 int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void *stream);
+/* hybrid boundary of the new field P (apply_abc): z lines, x lines, then in
+ * 3-D y lines, each face of a line swept by one thread */
+int awefd_launch_cart_hybrid(const awefd_cart_args *a, int nop, int ndim, void *stream);
 int awefd_launch_fill(float *p, long n, float v, void *stream);
 /* Batched 2-D step over many shots (cart_kernels.hip): shots = device array
  * of per-shot arguments, bshot = shot of every block, rx in {2,4,8,16}. */

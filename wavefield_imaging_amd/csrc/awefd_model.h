@@ -42,6 +42,17 @@ struct awefd_model {
 
     float *d_wav;
     long nwav;
+
+    /* cartesian modes: sponge = dabc without hybridbc (the sponge, and in
+     * 3-D the one-way faces); hyb = dabc with hybridbc (apply_abc); adj2 =
+     * 2-D adj=y (image at nt-kt, no traces); inj_rec = the injected points
+     * are the receivers (2-D adj=y without expl); nsteps = steps a run takes */
+    int sponge, hyb, adj2, inj_rec, nsteps;
+    float *d_damp_h;              /* damp table, nb - nop entries (hyb)      */
+    float *h_wav;                 /* adj=y: host copy of the wavelet, and    */
+    float *d_wavr;                /* its rows back to front on the device,   */
+    unsigned wrev_gen;            /* built for model generation wrev_gen     */
+    int wrev_ok;
     float *d_traces;
     long ntraces;
 

@@ -783,6 +783,76 @@ __global__ __launch_bounds__(256) void k_cart_abc_y(const awefd_cart_args a)
         if (in_sponge<3>(iz, ix, iy, a)) P[b + iy * sy] = sponge<3>(P[b + iy * sy], iz, ix, iy, a);
 }
 
+/* One face of one line of the hybrid boundary (apply_abc,
+ * MAWEFDcart2Dcpu.c:553-614, MAWEFDcart3Dcpu.c:534-630).  Node q of the face
+ * (q = 0 on the face, q+1 its inner neighbour) is at a0 + q st.  The nop
+ * outermost nodes take the one-way value, from q = nop-1 outward, starting
+ * from the stencil value at q = nop; then the nodes q = nb-1 .. nop blend the
+ * one-way value with their stencil value by damp[nb-1-q], each using its
+ * blended inner neighbour.  The inner neighbour's new value stays in a
+ * register; the stencil values are read before they are overwritten. */
+__device__ __forceinline__ void hyb_face(float *__restrict__ P, const float *__restrict__ C, long a0, long st,
+                                         int nop, int nb, const float *__restrict__ damp, float k)
+{
+    float in = P[a0 + nop * st];
+    for (int q = nop - 1; q >= 0; q--) {
+        const long i = a0 + q * st;
+        const float v = C[i + st] + (C[i] - in) * k;
+        P[i] = v;
+        in = v;
+    }
+    in = P[a0 + (long)nb * st];
+    for (int q = nb - 1; q >= nop; q--) {
+        const long i = a0 + q * st;
+        const float w = damp[nb - 1 - q];
+        const float bc = C[i + st] + (C[i] - in) * k;
+        const float v = bc * (1.f - w) + P[i] * w;
+        P[i] = v;
+        in = v;
+    }
+}
+
+/* Hybrid boundary along axis AX (0 z, 1 x, 2 y) of the new field P, C the
+ * current one.  Thread t < nline sweeps the low face of line t, the next
+ * nline threads the high faces, so neighbouring threads take neighbouring
+ * lines (consecutive z for the x and y lines).  The one-way tables are
+ * indexed by line: bz*[iy n2 + ix], bx*[iy n1 + iz], by*[ix n1 + iz].  A free
+ * surface zeroes z in [0, nb] of each z line after its low face. */
+template <int AX>
+__global__ __launch_bounds__(256) void k_cart_hybrid(const awefd_cart_args a, int nop)
+{
+    const long sx = a.ld, sy = (long)a.ld * a.n2;
+    const long nline = AX == 0 ? (long)a.n2 * a.n3 : AX == 1 ? (long)a.n1 * a.n3 : (long)a.n1 * a.n2;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= 2 * nline) return;
+    const bool hi = t >= nline;
+    const long line = hi ? t - nline : t;
+    long b, st;
+    int n;
+    const float *tab;
+    if (AX == 0) {
+        b = (line / a.n2) * sy + (line % a.n2) * sx;
+        st = 1;
+        n = a.n1;
+        tab = hi ? a.bzh : a.bzl;
+    } else if (AX == 1) {
+        b = (line / a.n1) * sy + line % a.n1;
+        st = sx;
+        n = a.n2;
+        tab = hi ? a.bxh : a.bxl;
+    } else {
+        b = (line / a.n1) * sx + line % a.n1;
+        st = sy;
+        n = a.n3;
+        tab = hi ? a.byh : a.byl;
+    }
+    const float k = tab[line];
+    if (hi) hyb_face(a.P, a.C, b + (long)(n - 1) * st, -st, nop, a.nb, a.damp, k);
+    else hyb_face(a.P, a.C, b, st, nop, a.nb, a.damp, k);
+    if (AX == 0 && !hi && a.fsrf)
+        for (int iz = 0; iz <= a.nb; iz++) a.P[b + iz] = 0.f;
+}
+
 __global__ void k_fill(float *p, long n, float v)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -965,6 +1035,17 @@ extern "C" int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void 
     case 6: return launch_abc<6>(a, s);
     default: return (int)hipErrorInvalidValue;
     }
+}
+
+extern "C" int awefd_launch_cart_hybrid(const awefd_cart_args *a, int nop, int ndim, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    const long nl[3] = {(long)a->n2 * a->n3, (long)a->n1 * a->n3, (long)a->n1 * a->n2};
+    hipLaunchKernelGGL(k_cart_hybrid<0>, dim3((unsigned)((2 * nl[0] + 255) / 256)), dim3(256), 0, s, *a, nop);
+    hipLaunchKernelGGL(k_cart_hybrid<1>, dim3((unsigned)((2 * nl[1] + 255) / 256)), dim3(256), 0, s, *a, nop);
+    if (ndim == 3)
+        hipLaunchKernelGGL(k_cart_hybrid<2>, dim3((unsigned)((2 * nl[2] + 255) / 256)), dim3(256), 0, s, *a, nop);
+    return (int)hipGetLastError();
 }
 
 extern "C" int awefd_launch_fill(float *p, long n, float v, void *stream)
