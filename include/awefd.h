@@ -72,6 +72,9 @@ typedef struct awefd_params {
                         boundary (apply_abc) replaces the sponge (and the 3-D
                         one-way faces); nb <= nop widens to 2 nop; a free surface
                         no longer zeroes (v dt)^2 (MAWEFDcart2Dcpu.c:211-212,276-292) */
+    int   sinc;      /* sinc=  cartesian: sources and receivers on 8-point sinc
+                        taps, 9 nodes per axis (sinc2d_make/sinc3d_make,
+                        fdutil.c:784-1360) instead of bilinear corners      */
 } awefd_params;
 
 typedef struct awefd_model awefd_model;

@@ -194,6 +194,167 @@ void orc_lint2d(int na, const float *xz, int nzpad, int nxpad, float ozpad,
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* 8-point sinc weights of sinc2d_make / sinc3d_make (fdutil.c:784-842,
+ * 1088-1145).  They come from ints8r, the CWP/SU library's 8-point
+ * least-squares sinc interpolator (Dave Hale; intsinc8.c with mksinc.c and
+ * the Levinson solver of stoep.c), which Madagascar builds from its su tree;
+ * it is not part of the reference.  Restated from that published method:
+ *   - mksinc(d): the 8 coefficients s minimising the error of
+ *     y(i+d) ~ sum_j s_j y(i+j-3) over frequencies up to
+ *     fmax = min(0.066 + 0.265 ln 8, 1) of Nyquist, i.e. the symmetric
+ *     Toeplitz system sum_k sinc(fmax (j-k)) s_k = sinc(fmax (3 - j + d)),
+ *     solved in double by Levinson's recursion;
+ *   - a table of 513 fractional shifts d = k/512 (k = 0 and 512: the unit
+ *     impulse on the 4th / 5th tap);
+ *   - intt8r: output x -> position xn = 8 + (x - x0)/dx, taps
+ *     yin[int(xn) - 11 ...], table row round(frac(xn) 512), samples off
+ *     the input taken as yinl / yinr.
+ * Parity with the reference is unpinned for these weights: the tests pin
+ * the method's own properties (the Toeplitz system, symmetry, impulse
+ * rows, interpolation accuracy). */
+#define ORC_SINC_L 8
+#define ORC_SINC_NT 513
+
+static double orc_dsinc(double x)
+{
+    if (x == 0.0) return 1.0;
+    double px = 3.141592653589793 * x;
+    return sin(px) / px;
+}
+
+/* Levinson recursion for R f = g, R symmetric Toeplitz with top row r
+ * (Claerbout, FGDP p. 57): a = prediction-error filter, v its power. */
+static void orc_levinson(int n, const double *r, const double *g, double *f, double *a)
+{
+    if (r[0] == 0.0) return;
+    a[0] = 1.0;
+    double v = r[0];
+    f[0] = g[0] / r[0];
+    for (int j = 1; j < n; j++) {
+        a[j] = 0.0;
+        f[j] = 0.0;
+        double e = 0.0;
+        for (int i = 0; i < j; i++) e += a[i] * r[j - i];
+        double c = e / v;
+        v -= c * e;
+        for (int i = 0; i <= j / 2; i++) {
+            double hi = a[j - i] - c * a[i];
+            a[i] -= c * a[j - i];
+            a[j - i] = hi;
+        }
+        double w = 0.0;
+        for (int i = 0; i < j; i++) w += f[i] * r[j - i];
+        c = (w - g[j]) / v;
+        for (int i = 0; i <= j; i++) f[i] -= c * a[j - i];
+    }
+}
+
+void orc_mksinc(float d, float *out)
+{
+    double r[20], g[20], f[20], a[20];
+    double fmax = 0.066 + 0.265 * log((double)ORC_SINC_L);
+    if (fmax > 1.0) fmax = 1.0;
+    for (int j = 0; j < ORC_SINC_L; j++) {
+        r[j] = orc_dsinc(fmax * j);
+        g[j] = orc_dsinc(fmax * (ORC_SINC_L / 2 - j - 1 + d));
+    }
+    orc_levinson(ORC_SINC_L, r, g, f, a);
+    for (int j = 0; j < ORC_SINC_L; j++) out[j] = (float)f[j];
+}
+
+static float orc_stab[ORC_SINC_NT][ORC_SINC_L];
+static int orc_stab_ok;
+
+static void orc_sinc_tabulate(void)
+{
+    if (orc_stab_ok) return;
+    for (int k = 1; k < ORC_SINC_NT - 1; k++)
+        orc_mksinc((float)k / (float)(ORC_SINC_NT - 1), orc_stab[k]);
+    for (int j = 0; j < ORC_SINC_L; j++) orc_stab[0][j] = orc_stab[ORC_SINC_NT - 1][j] = 0.f;
+    orc_stab[0][ORC_SINC_L / 2 - 1] = 1.f;
+    orc_stab[ORC_SINC_NT - 1][ORC_SINC_L / 2] = 1.f;
+    orc_stab_ok = 1;
+}
+
+void orc_ints8r(int nxin, float dxin, float fxin, const float *yin, float yinl, float yinr, int nxout,
+                const float *xout, float *yout)
+{
+    orc_sinc_tabulate();
+    float xs = 1.0f / dxin, xb = 8.0f - fxin * xs, fnm1 = (float)(ORC_SINC_NT - 1);
+    for (int i = 0; i < nxout; i++) {
+        float xn = xb + xout[i] * xs;
+        int ixn = (int)xn;
+        int k = -11 + ixn;
+        float frac = xn - (float)ixn;
+        int kt = frac >= 0.0 ? (int)(frac * fnm1 + 0.5) : (int)((frac + 1.0) * fnm1 - 0.5);
+        const float *t = orc_stab[kt];
+        if (k >= 0 && k <= nxin - 8) {
+            yout[i] = yin[k] * t[0] + yin[k + 1] * t[1] + yin[k + 2] * t[2] + yin[k + 3] * t[3]
+                      + yin[k + 4] * t[4] + yin[k + 5] * t[5] + yin[k + 6] * t[6] + yin[k + 7] * t[7];
+        } else {
+            float sum = 0.0f;
+            for (int j = 0; j < 8; j++, k++) {
+                float y = k < 0 ? yinl : k >= nxin ? yinr : yin[k];
+                sum += y * t[j];
+            }
+            yout[i] = sum;
+        }
+    }
+}
+
+/* One axis of a sinc point: q = (node - point)/h as the reference forms it
+ * (in float); the z axis builds its offsets in double (fdutil.c:834,1131:
+ * -4.0 + dz/dz + i), x and y in float.  w = response of ints8r to a unit
+ * impulse at the centre tap; a weight of exactly 1 at the centre (the point
+ * on a node) keeps that tap only: *f = 4, *n = 1. */
+void orc_sinc_axis(float q, int zaxis, float *w, int *f, int *n)
+{
+    float inp[9] = {0, 0, 0, 0, 1, 0, 0, 0, 0}, xo[9];
+    for (int i = 0; i < 9; i++)
+        xo[i] = zaxis ? (float)(-4.0 + (double)q + (double)(i * 1.0f)) : -4.0f + q + i * 1.0f;
+    orc_ints8r(9, 1.0f, -4.0f, inp, 0.0f, 0.0f, 9, xo, w);
+    *f = 0;
+    *n = 9;
+    if (w[4] == 1.0f) {
+        *f = 4;
+        *n = 1;
+    }
+}
+
+/* Centre node and taps of a point along one axis (fdutil.c:802-808,
+ * 1110-1117): i = (int)((c - o)/h + 0.499), q = (i h + o - c)/h. */
+static void orc_sinc_node(float c, float o, float h, int zaxis, int *i, float *w, int *f, int *n)
+{
+    *i = (int)((c - o) / h + 0.499f);
+    float dd = *i * h + o - c;
+    orc_sinc_axis(dd / h, zaxis, w, f, n);
+}
+
+/* Sinc records of na points, 2-D coordinates (x,z) or 3-D (x,y,z): per
+ * point rec[0..2] = centre (z, x, y), rec[3..8] = (fz, nz, fx, nx, fy, ny),
+ * w[27] = z, x, y weights.  Returns -1 when a tap leaves the padded grid. */
+int orc_sinc_make(int ndim, int na, const float *c, const int np[3], const float opad[3], const float h[3],
+                  int *rec, float *w)
+{
+    int bad = 0;
+    for (int a = 0; a < na; a++) {
+        const float *p = c + (long)ndim * a;
+        int *r = rec + 9 * a;
+        float *wa = w + 27 * a;
+        /* the reference builds y, then x, then z (fdutil.c:802-839) */
+        if (ndim == 3) orc_sinc_node(p[1], opad[2], h[2], 0, &r[2], wa + 18, &r[7], &r[8]);
+        else { r[2] = 0; r[7] = 4; r[8] = 1; for (int i = 0; i < 9; i++) wa[18 + i] = i == 4; }
+        orc_sinc_node(p[0], opad[1], h[1], 0, &r[1], wa + 9, &r[5], &r[6]);
+        orc_sinc_node(p[ndim - 1], opad[0], h[0], 1, &r[0], wa, &r[3], &r[4]);
+        for (int ax = 0; ax < ndim; ax++) {
+            int lo = r[ax] - 4 + r[3 + 2 * ax], hi = lo + r[4 + 2 * ax] - 1;
+            if (lo < 0 || hi >= np[ax]) bad = 1;
+        }
+    }
+    return bad ? -1 : 0;
+}
+
 void orc_lint3d(int na, const float *xyz, int nzpad, int nxpad, int nypad,
                 float ozpad, float oxpad, float oypad, float dz, float dx, float dy,
                 int *jz, int *jx, int *jy, float *w)

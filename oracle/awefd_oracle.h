@@ -58,6 +58,7 @@ typedef struct {
                         injects nr samples per step at the receivers          */
     int   kt;        /* cartesian 2-D kt= (image step nt-kt)                   */
     int   hybrid;    /* cartesian hybridbc=: apply_abc instead of the sponge   */
+    int   sinc;      /* cartesian sinc=: 8-point sinc injection and extraction */
 } orc_par;
 
 /* Snapshot callback: padded field (the reference's u0 / h_po after the step),
@@ -102,6 +103,15 @@ int orc_spher3d_f64(const orc_par *p, const float *vel,
 /* Wall time (s) of the last time loop, and the OpenMP thread count. */
 double orc_last_loop_seconds(void);
 int orc_last_imaged(void);
+
+/* ints8r and the sinc point records of sinc2d_make/sinc3d_make (see
+ * awefd_oracle.c) */
+void orc_mksinc(float d, float *out);
+void orc_ints8r(int nxin, float dxin, float fxin, const float *yin, float yinl, float yinr, int nxout,
+                const float *xout, float *yout);
+void orc_sinc_axis(float q, int zaxis, float *w, int *f, int *n);
+int  orc_sinc_make(int ndim, int na, const float *c, const int np[3], const float opad[3], const float h[3],
+                   int *rec, float *w);
 int    orc_threads(void);
 
 /* Building blocks exposed for the known-answer tests. */

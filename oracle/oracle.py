@@ -24,7 +24,7 @@ class OrcPar(C.Structure):
         ("fdorder", C.c_int), ("optfd", C.c_int), ("dabc", C.c_int),
         ("fsrf", C.c_int), ("expl", C.c_int), ("cden", C.c_int), ("sse", C.c_int),
         ("ns", C.c_int), ("nr", C.c_int), ("snap", C.c_int), ("jsnap", C.c_int),
-        ("adj", C.c_int), ("kt", C.c_int), ("hybrid", C.c_int),
+        ("adj", C.c_int), ("kt", C.c_int), ("hybrid", C.c_int), ("sinc", C.c_int),
     ]
 
 
@@ -152,6 +152,7 @@ def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, sn
     p.adj = int(_yn(par.get("adj", False)))
     p.kt = int(par.get("kt", 0))
     p.hybrid = int(_yn(par.get("hybridbc", False)))
+    p.sinc = int(_yn(par.get("sinc", False))) if kind.startswith("cart") else 0
     ndim = 3 if kind.endswith("3d") else 2
     sou = np.ascontiguousarray(sou, np.float32).reshape(-1)
     rec = np.ascontiguousarray(rec, np.float32).reshape(-1)

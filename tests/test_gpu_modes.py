@@ -108,3 +108,39 @@ def test_adj_and_hybrid_errors(orc):
             awefd.Batch([m])
     for m in ms:
         m.close()
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(expl=True), dict(fdorder=8, sse=False), dict(hybridbc=True, free=True),
+                                dict(adj=True, kt=9), dict(adj=True, kt=30, expl=True), dict(dabc=False)])
+def test_cart2d_sinc_bit_exact(orc, kw):
+    """sinc=y: 9 x 9 taps per point; overlapping taps of nearby sources add
+    in source order; on-node and off-node points."""
+    par, vel, sou, rec, wav = _case2d(nt=110, nb=10, sinc=True, **kw)
+    sou = np.float32([[101.3, 62.2], [104.0, 60.0], [140.7, 33.9]])
+    rec = np.float32([[x, 41.3 + 0.37 * i] for i, x in enumerate(np.arange(45.0, 240.0, 13.1))])
+    adj2 = kw.get("adj", False)
+    k = 1 if kw.get("expl") else (len(rec) if adj2 else len(sou))
+    rng = np.random.default_rng(11)
+    w = (wav[:, None] * rng.standard_normal(k)[None, :]).astype(np.float32)
+    ref, _ = _run_both("cart2d", par, vel, sou, rec, w, orc, jsnap=17)
+    if not adj2:
+        assert np.abs(ref).max() > 0
+
+
+@pytest.mark.parametrize("kw", [dict(), dict(expl=True, free=True), dict(hybridbc=True), dict(adj=True)])
+def test_cart3d_sinc_bit_exact(orc, kw):
+    par, vel, sou, rec, wav = _case3d(nt=50, nb=7, sinc=True, **kw)
+    sou = np.float32([[71.3, 72.9, 33.4], [74.0, 70.0, 35.0]])
+    rec = rec[::4] + np.float32([[0.7, -1.3, 2.1]])
+    k = 1 if kw.get("expl") else len(sou)
+    w = (wav[:, None] * np.float32([1.0, -0.6])[None, :k]).astype(np.float32)
+    _run_both("cart3d", par, vel, sou, rec, w, orc, jsnap=21)
+
+
+def test_sinc_taps_must_stay_on_the_grid(orc):
+    par, vel, sou, rec, wav = _case2d(nt=20, nb=2, sinc=True)
+    with awefd.Model("cart2d", par) as m:
+        m.set_model(vel)
+        with pytest.raises(awefd.L.AwefdError):
+            m.set_sources(np.float32([[3.0, 90.0]]))      # 4 taps toward x < 0 leave the grid
+        m.set_sources(np.float32([[0.0, 90.0]]))          # on a node: one tap

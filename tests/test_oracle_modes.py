@@ -11,6 +11,11 @@ hybridbc=y (apply_abc, MAWEFDcart2Dcpu.c:553-614, MAWEFDcart3Dcpu.c:534-630):
 * absorption against a domain large enough that no echo returns in time;
 * the nb rule (nb <= nop widens to 2 nop).
 
+sinc=y (fdutil.c:784-1360): the tap weights against the least-squares
+system that defines them and the interpolator's published accuracy (their
+parity with the reference is unpinned: ints8r is not in the reference); a
+sinc point against on-node points on its taps, bit for bit.
+
 adj=y (MAWEFDcart2Dcpu.c:297-378, MAWEFDcart3Dcpu.c:295-360): against the
 forward mode, which the other oracle tests pin:
 * 3-D: the traces of a forward run with the wavelet rows reversed;
@@ -265,3 +270,152 @@ def test_adj2d_kt_never_reached(orc):
     tr, _, _ = orc.run("cart2d", par, vel, np.float32([[50.0, 50.0]]), np.float32([[40.0, 10.0]]),
                        synth.ricker(nt, synth.DT))
     assert not orc.imaged() and np.all(tr == 0)
+
+
+# --------------------------------------------------------------------------
+# sinc=y (sinc2d_make / sinc3d_make and their inject / extract,
+# fdutil.c:784-1360).  The tap weights come from ints8r, the CWP/SU 8-point
+# sinc interpolator, which the reference does not carry: their parity with
+# the reference is unpinned.  Pinned here: the method's defining system and
+# its published accuracy, and the wiring of the taps into the loop.
+
+def _fp(a):
+    import ctypes as C
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _mksinc(orc, d):
+    import ctypes as C
+    out = np.zeros(8, np.float32)
+    orc.lib().orc_mksinc(C.c_float(d), _fp(out))
+    return out
+
+
+def _axis(orc, q, zaxis=False):
+    import ctypes as C
+    w = np.zeros(9, np.float32)
+    f, n = C.c_int(), C.c_int()
+    orc.lib().orc_sinc_axis(C.c_float(q), int(zaxis), _fp(w), C.byref(f), C.byref(n))
+    return w, f.value, n.value
+
+
+def test_sinc_taps_solve_the_least_squares_system(orc):
+    """mksinc: the normal equations of the least-squares fit of sinc over
+    [0, fmax], fmax = 0.066 + 0.265 ln 8 of Nyquist, solved independently."""
+    from scipy.linalg import solve_toeplitz
+
+    from scipy.linalg import toeplitz
+
+    fmax = 0.066 + 0.265 * np.log(8.0)
+    j = np.arange(8)
+    R = toeplitz(np.sinc(fmax * j))       # condition number ~1e6: compare residuals
+    for d in (1 / 512, 0.1, 0.25, 0.5, 0.73, 511 / 512):
+        # the shifts 3 - j + d are formed in float, as mksinc's float d makes them
+        g = np.sinc(fmax * ((3 - j).astype(np.float32) + np.float32(d)).astype(np.float64))
+        s = solve_toeplitz(np.sinc(fmax * j), g)
+        mine = _mksinc(orc, d).astype(np.float64)
+        assert np.abs(R @ mine - g).max() < 2e-7
+        np.testing.assert_allclose(mine, s, rtol=0, atol=2e-7)
+        # a shift of d mirrors a shift of 1 - d (up to the float shifts)
+        np.testing.assert_allclose(_mksinc(orc, d), _mksinc(orc, 1 - d)[::-1], rtol=0, atol=5e-6)
+
+
+def test_ints8r_accuracy(orc):
+    """ints8r's published accuracy: below 1% error for frequencies up to 0.6
+    of Nyquist; samples off the input are the given end values."""
+    import ctypes as C
+
+    n = 64
+    x = np.arange(n, dtype=np.float32)
+    rng = np.random.default_rng(3)
+    xo = (8 + rng.uniform(0, n - 16, 400)).astype(np.float32)
+    for f in (0.05, 0.15, 0.3):                      # cycles per sample; Nyquist 0.5
+        ph = rng.uniform(0, 2 * np.pi)
+        y = np.cos(2 * np.pi * f * x + ph).astype(np.float32)
+        out = np.zeros(xo.size, np.float32)
+        orc.lib().orc_ints8r(n, C.c_float(1.0), C.c_float(0.0), _fp(y), C.c_float(0), C.c_float(0), xo.size,
+                             _fp(xo), _fp(out))
+        err = np.abs(out - np.cos(2 * np.pi * f * xo.astype(np.float64) + ph)).max()
+        assert err < 0.01, (f, err)
+    # on the input samples: exact
+    out = np.zeros(n, np.float32)
+    y = rng.standard_normal(n).astype(np.float32)
+    orc.lib().orc_ints8r(n, C.c_float(1.0), C.c_float(0.0), _fp(y), C.c_float(7), C.c_float(-7), n, _fp(x),
+                         _fp(out))
+    np.testing.assert_array_equal(out, y)
+
+
+def test_sinc_axis_weights(orc):
+    """On a node: one tap of weight 1.  Off a node: 9 taps summing to ~1 (the
+    fit is least-squares, not interpolating a constant exactly), the largest
+    on the nearest node, the stencil mirrored for a mirrored offset."""
+    w, f, n = _axis(orc, 0.0)
+    assert (f, n) == (4, 1) and w[4] == 1 and np.count_nonzero(w) == 1
+    for q in (0.1, 0.25, 0.4, 0.5):
+        w, f, n = _axis(orc, q)
+        wm, _, _ = _axis(orc, -q)
+        assert (f, n) == (0, 9)
+        assert abs(w.sum() - 1) < 3e-3
+        assert np.argmax(w) == 4 or (q == 0.5 and np.argmax(w) in (3, 4))
+        np.testing.assert_allclose(w[:8], wm[1:][::-1], atol=3e-7)
+        # the point sits at node offset -q: weights fall off with distance
+        assert w[3] > w[5] if q > 0 else True
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_sinc_point_is_its_taps(orc, ndim):
+    """A sinc source equals on-node (single-tap) sources on its tap nodes,
+    each fed the wavelet times its tap weights in the reference's order,
+    ((w wy) wx) wz, bit for bit; a sinc receiver's trace equals the sum,
+    in tap order y, x, z, of ((u wy) wx) wz over on-node receivers on the
+    same nodes (u: their traces)."""
+    nt, nb, h = 40, 6, 5.0
+    n = [24, 26, 22][:ndim]
+    vel = synth.layered_3d(24, 26, 22, seed=2, air=4, base_mean=14, amp=2.0, corr=4.0) if ndim == 3 else \
+        synth.layered_2d(24, 26, seed=2, air=4, base_mean=14, amp=2.0, corr=4.0)
+    # point (x, [y,] z) off the nodes on every axis
+    pt = [57.3, 48.9, 61.7][:ndim] if ndim == 3 else [57.3, 61.7]
+    recp = [[31.1, 70.2, 44.4][:ndim]] if ndim == 3 else [[31.1, 44.4]]
+    wav = synth.ricker(nt, synth.DT, freq=2e7, delay=0.0)
+    kind = "cart%dd" % ndim
+    par = dict(n=n, o=[0.0] * ndim, d=[h] * ndim, nt=nt, dt=synth.DT, nb=nb, dabc=True, expl=False)
+    tr_s, f_s, _ = orc.run(kind, dict(par, sinc=True), vel, np.float32([pt]), np.float32(recp), wav)
+
+    def taps(p):
+        # (x, [y,] z) -> per-axis (first node index in model coordinates, weights)
+        out = {}
+        names = ["x", "y", "z"] if ndim == 3 else ["x", "z"]
+        for ax, c in zip(names, p):
+            i = int(np.float32(np.float32(c) - np.float32(-nb * h)) / np.float32(h) + np.float32(0.499))
+            q = np.float32(np.float32(np.float32(i) * np.float32(h) + np.float32(-nb * h)) - np.float32(c))
+            w, f, m = _axis(orc, np.float32(q / np.float32(h)), zaxis=ax == "z")
+            out[ax] = [((i - 4 + t) * h - nb * h, w[t]) for t in range(f, f + m)]
+        return out
+
+    t = taps(pt)
+    order = ([(y, x, z) for y in t["y"] for x in t["x"] for z in t["z"]] if ndim == 3 else
+             [(x, z) for x in t["x"] for z in t["z"]])
+    coords = lambda o: [o[1][0], o[0][0], o[2][0]] if ndim == 3 else [o[0][0], o[1][0]]   # (x, [y,] z)
+    nodes = np.float32([coords(o) for o in order])
+    amps = []
+    for o in order:
+        a = wav.astype(np.float32)
+        for _, w in o:
+            a = (a * np.float32(w)).astype(np.float32)
+        amps.append(a)
+    tr_n, f_n, _ = orc.run(kind, par, vel, nodes, np.float32(recp), np.stack(amps, 1))
+    np.testing.assert_array_equal(f_s, f_n)
+    # receiver side
+    r = taps(recp[0])
+    rorder = ([(y, x, z) for y in r["y"] for x in r["x"] for z in r["z"]] if ndim == 3 else
+              [(x, z) for x in r["x"] for z in r["z"]])
+    rnodes = np.float32([coords(o) for o in rorder])
+    u, _, _ = orc.run(kind, dict(par, sinc=True), vel, np.float32([pt]), rnodes, wav)
+    g = np.zeros(nt, np.float32)
+    for k, o in enumerate(rorder):
+        term = u[:, k].astype(np.float32)
+        for _, w in o:
+            term = (term * np.float32(w)).astype(np.float32)
+        g = (g + term).astype(np.float32)
+    np.testing.assert_array_equal(tr_s[:, 0], g)
+    assert np.abs(g).max() > 0

@@ -38,7 +38,7 @@ class Params(C.Structure):
         ("nt", C.c_int), ("dt", C.c_float), ("nb", C.c_int), ("fdorder", C.c_int),
         ("optfd", C.c_int), ("dabc", C.c_int), ("fsrf", C.c_int), ("expl", C.c_int),
         ("cden", C.c_int), ("sse", C.c_int), ("jdata", C.c_int), ("device", C.c_int),
-        ("adj", C.c_int), ("kt", C.c_int), ("hybrid", C.c_int),
+        ("adj", C.c_int), ("kt", C.c_int), ("hybrid", C.c_int), ("sinc", C.c_int),
     ]
 
 

@@ -29,7 +29,7 @@ def make_params(kind, par):
     """Build an awefd_params from program parameters.
 
     par: n (list), o, d, nt, dt and any getpar key (nb, fdorder, optfd, dabc,
-    free/fsrf, expl, cden, sse, jdata, gpu, adj, kt, hybridbc).
+    free/fsrf, expl, cden, sse, jdata, gpu, adj, kt, hybridbc, sinc).
     """
     k = L.KIND_NAMES[kind] if isinstance(kind, str) else int(kind)
     lib = L.load()
@@ -63,6 +63,8 @@ def make_params(kind, par):
         p.kt = int(par["kt"])
     if "hybridbc" in par:
         p.hybrid = int(_yn(par["hybridbc"]))
+    if "sinc" in par:
+        p.sinc = int(_yn(par["sinc"]))
     p.device = int(par.get("gpu", 0))
     return p
 

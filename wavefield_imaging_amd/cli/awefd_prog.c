@@ -143,7 +143,6 @@ int main(int argc, char **argv)
         par_bool("hybridbc", &hybrid);
         par_bool("sinc", &sinc);
         par_int("kt", &kt);   /* MAWEFDcart2Dcpu.c:165-167: 0 when absent */
-        if (sinc) die("sinc=y is not available in this build");
     } else {
         /* MAWEFDspher2Dcpu.c:56-60 (snap, bnds default y), MAWEFDspher3Dcpu.c:57-61 */
         snap = ndim == 2;
@@ -194,6 +193,7 @@ int main(int argc, char **argv)
     p.adj = cart && adj;
     p.kt = kt;
     p.hybrid = cart && hybrid;
+    p.sinc = cart && sinc;
     /* 2-D adj=y: the output is the image at step nt-kt, not traces */
     const int image = kind == AWEFD_CART2D && adj;
 

@@ -95,6 +95,8 @@ static int point_buckets(const awefd_model *m, int n, const int *j, const float 
                 k->e.y = 0;
                 k->e.src = same_id ? 0 : a;
                 k->e.w = w[8 * a + q];
+                k->e.w2 = 1.f;
+                k->e.w3 = 1.f;
                 k->bucket = k->e.z / 64 + ntz * (k->e.x / AWEFD_ADJ_TX);
             }
         awefd_sort_inj(keys, ne, nbucket, e, beg);
@@ -115,8 +117,8 @@ static int check_model(awefd_model *m, const char *who)
     if (m->p.kind != AWEFD_CART2D) return awefd__fail(AWEFD_EINVAL, "%s: cartesian 2-D models only", who);
     if (!m->p.cden && m->drho)
         return awefd__fail(AWEFD_EINVAL, "%s: constant density only (cden=y or no density set)", who);
-    if (m->p.adj || m->hyb)
-        return awefd__fail(AWEFD_EINVAL, "%s: adj=y and hybridbc=y loops have no gradient", who);
+    if (m->p.adj || m->hyb || m->sinc)
+        return awefd__fail(AWEFD_EINVAL, "%s: adj=y, hybridbc=y and sinc=y loops have no gradient", who);
     if (!m->model_set || !m->src_set || !m->rec_set || !m->wav_set)
         return awefd__fail(AWEFD_ESTATE, "%s: model not fully set up", who);
     const long need = (long)m->p.nt * (m->p.expl ? 1 : m->ns);

@@ -85,7 +85,7 @@ void awefd_destroy(awefd_model *m)
                      (void **)&m->d_wav,  (void **)&m->d_traces, (void **)&m->d_sj,
                      (void **)&m->d_sw,   (void **)&m->d_srefl, (void **)&m->d_damp,
                      (void **)&m->d_metric, (void **)&m->d_abcrules, (void **)&m->d_damp_h,
-                     (void **)&m->d_wavr};
+                     (void **)&m->d_wavr, (void **)&m->d_rsinc};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; i++) free_dev(ptrs[i]);
     if (m->ev) {
         for (int i = 0; i < m->nev; i++) hipEventDestroy(m->ev[i]);
@@ -96,6 +96,8 @@ void awefd_destroy(awefd_model *m)
     free(m->h_sp[0]); free(m->h_sp[1]); free(m->h_sp[2]);
     free(m->h_srefl);
     free(m->h_wav);
+    free(m->h_ssinc);
+    free(m->h_rsinc);
     awefd__spher_free(m);
     awefd__adjoint_free(m);
     if (m->cgexec) hipGraphExecDestroy(m->cgexec);
@@ -189,6 +191,7 @@ int awefd_create(const awefd_params *p, awefd_model **out)
         m->hyb = p->dabc && p->hybrid;
         m->sponge = p->dabc && !p->hybrid;
         if (m->hyb && m->nb <= m->nop) m->nb = 2 * m->nop;
+        m->sinc = p->sinc != 0;
         m->adj2 = p->adj && m->ndim == 2;
         m->inj_rec = m->adj2 && !p->expl;
         m->nsteps = p->nt;
@@ -452,9 +455,22 @@ static int set_points(awefd_model *m, int which, int n, const float *coords)
         }
     }
     awefd_lint(m->ndim, n, coords, npad, opad, d, j, w);
+    awefd_sinc_pt *sp = NULL;
+    if (m->sinc) {
+        /* sinc=y: the 9-node taps replace the bilinear corners */
+        sp = malloc(sizeof(*sp) * (n > 0 ? n : 1));
+        if (!sp || awefd_sinc_points(m->ndim, n, coords, npad, opad, d, sp)) {
+            free(j);
+            free(w);
+            free(sp);
+            return sp ? awefd__fail(AWEFD_EINVAL, "a %s's sinc taps (4 nodes each way) leave the padded "
+                                    "grid", which == 0 ? "source" : "receiver")
+                      : awefd__fail(AWEFD_ENOMEM, "sinc records");
+        }
+    }
     /* every interpolation corner must be a padded node: the kernels index
      * with these values unchecked */
-    for (int a = 0; a < n; a++)
+    for (int a = 0; a < n && !m->sinc; a++)
         for (int ax = 0; ax < m->ndim; ax++)
             if (j[3 * a + ax] < 0 || j[3 * a + ax] + 1 >= npad[ax]) {
                 free(j);
@@ -462,6 +478,13 @@ static int set_points(awefd_model *m, int which, int n, const float *coords)
                 return awefd__fail(AWEFD_EINVAL, "%s %d maps outside the padded grid",
                                    which == 0 ? "source" : "receiver", a);
             }
+    if (which == 0) {
+        free(m->h_ssinc);
+        m->h_ssinc = sp;
+    } else {
+        free(m->h_rsinc);
+        m->h_rsinc = sp;
+    }
     if (which == 0) {
         free(m->h_sj); free(m->h_sw);
         m->h_sj = j; m->h_sw = w; m->ns = n;
@@ -483,7 +506,12 @@ int awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj
     const int npt = m->inj_rec ? m->nr : m->ns;
     const int *pj = m->inj_rec ? m->h_rj : m->h_sj;
     const float *pw = m->inj_rec ? m->h_rw : m->h_sw;
-    const long n = (long)npt * nc;
+    const awefd_sinc_pt *psp = m->inj_rec ? m->h_rsinc : m->h_ssinc;
+    long n = (long)npt * nc;
+    if (m->sinc) {
+        n = 0;
+        for (int s = 0; s < npt; s++) n += (long)psp[s].n[0] * psp[s].n[1] * psp[s].n[2];
+    }
     const int ntx = (m->np[1] + tx - 1) / tx;
     const int nbucket = ntz * ntx;
     awefd_inj_key *keys = malloc(sizeof(*keys) * (n > 0 ? n : 1));
@@ -493,7 +521,7 @@ int awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj
         free(keys); free(e); free(ibeg);
         return awefd__fail(AWEFD_ENOMEM, "injection buckets");
     }
-    for (int s = 0; s < npt; s++)
+    for (int s = 0; s < npt && !m->sinc; s++)
         for (int q = 0; q < nc; q++) {
             awefd_inj_key *k = &keys[(long)s * nc + q];
             k->e.z = pj[3 * s] + (q & 1);
@@ -501,8 +529,30 @@ int awefd__cart_buckets(const awefd_model *m, int tz, int ntz, int tx, awefd_inj
             k->e.y = m->ndim == 3 ? pj[3 * s + 2] + ((q >> 2) & 1) : 0;
             k->e.src = s;
             k->e.w = pw[8 * s + q];
+            k->e.w2 = 1.f;
+            k->e.w3 = 1.f;
             k->bucket = (k->e.z / tz) + ntz * (k->e.x / tx);
         }
+    /* sinc taps: node (c - 4 + t) per axis receives ((amp wy) wx) wz v in
+     * 3-D, (amp wx) wz v in 2-D (sinc*d_inject*_with_vv, fdutil.c:888-1012,
+     * 1184-1298) */
+    long q = 0;
+    for (int s = 0; s < npt && m->sinc; s++) {
+        const awefd_sinc_pt *p = &psp[s];
+        for (int ty = p->f[2]; ty < p->f[2] + p->n[2]; ty++)
+            for (int tx_ = p->f[1]; tx_ < p->f[1] + p->n[1]; tx_++)
+                for (int tz_ = p->f[0]; tz_ < p->f[0] + p->n[0]; tz_++) {
+                    awefd_inj_key *k = &keys[q++];
+                    k->e.z = p->c[0] - 4 + tz_;
+                    k->e.x = p->c[1] - 4 + tx_;
+                    k->e.y = m->ndim == 3 ? p->c[2] - 4 + ty : 0;
+                    k->e.src = s;
+                    k->e.w = m->ndim == 3 ? p->w[2][ty] : p->w[1][tx_];
+                    k->e.w2 = m->ndim == 3 ? p->w[1][tx_] : p->w[0][tz_];
+                    k->e.w3 = m->ndim == 3 ? p->w[0][tz_] : 1.f;
+                    k->bucket = (k->e.z / tz) + ntz * (k->e.x / tx);
+                }
+    }
     awefd_sort_inj(keys, n, nbucket, e, ibeg);
     free(keys);
     *inj = e;
@@ -545,6 +595,8 @@ int awefd_set_receivers(awefd_model *m, int nr, const float *coords)
     HIPCHK(hipSetDevice(m->dev));
     rc = upload(m, (void **)&m->d_rj, m->h_rj, sizeof(int) * 3 * (nr > 0 ? nr : 1));
     if (!rc) rc = upload(m, (void **)&m->d_rw, m->h_rw, sizeof(float) * 8 * (nr > 0 ? nr : 1));
+    if (!rc && m->sinc)
+        rc = upload(m, (void **)&m->d_rsinc, m->h_rsinc, sizeof(awefd_sinc_pt) * (nr > 0 ? nr : 1));
     if (!rc && m->inj_rec) rc = cart_build_inj(m);
     if (!rc) m->rec_set = 1;
     m->gen++;
@@ -735,6 +787,7 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
     a->nr = m->adj2 ? 0 : m->nr;     /* the 2-D adj=y loop extracts no traces */
     a->rj = m->d_rj;
     a->rw = m->d_rw;
+    a->rsinc = m->sinc ? m->d_rsinc : NULL;
 }
 
 static int check_ready(awefd_model *m)

@@ -168,8 +168,8 @@ int awefd_batch_create(int nshot, awefd_model *const *models, awefd_batch **out)
             return awefd__fail(AWEFD_EINVAL, "batch: shot %d differs from shot 0 in nt, fdorder, sse or device", s);
         if (m->drho && !m->p.cden)
             return awefd__fail(AWEFD_EINVAL, "batch: shot %d has variable density (not batched)", s);
-        if (m->p.adj || m->hyb)
-            return awefd__fail(AWEFD_EINVAL, "batch: shot %d uses adj=y or hybridbc=y (not batched)", s);
+        if (m->p.adj || m->hyb || m->sinc)
+            return awefd__fail(AWEFD_EINVAL, "batch: shot %d uses adj=y, hybridbc=y or sinc=y (not batched)", s);
         if (!m->model_set || !m->src_set || !m->rec_set || !m->wav_set)
             return awefd__fail(AWEFD_ESTATE, "batch: shot %d is not fully set up", s);
         const long need = (long)m->p.nt * (m->p.expl ? 1 : m->ns);
@@ -390,6 +390,8 @@ static int t2_buckets(const awefd_batch *b, const awefd_model *m, int ntz2, int 
                     k->e.y = 0;
                     k->e.src = s;
                     k->e.w = m->h_sw[8 * s + q];
+                    k->e.w2 = 1.f;
+                    k->e.w3 = 1.f;
                     k->bucket = tz + ntz2 * tx;
                 }
             }

@@ -26,8 +26,19 @@ extern "C" {
 typedef struct {
     int   z, x, y;
     int   src;
-    float w;
+    float w, w2, w3;   /* the node receives ((amp w) w2) w3 v: bilinear
+                          points w2 = w3 = 1; sinc points the per-axis tap
+                          weights (2-D wx, wz, 1; 3-D wy, wx, wz) */
 } awefd_inj;
+
+/* 8-point sinc point (sinc2d_make / sinc3d_make, fdutil.c:784-842,1088-1145):
+ * centre node, first tap (0..8) and tap count (9, or 1 on a node) per axis,
+ * and the taps' weights; axis order z, x, y. */
+typedef struct {
+    int   c[3];
+    int   f[3], n[3];
+    float w[3][9];
+} awefd_sinc_pt;
 
 /* Arguments of the cartesian step kernels. */
 typedef struct {
@@ -65,6 +76,7 @@ typedef struct {
     int nr;
     const int *rj;           /* 3 ints per receiver (jz, jx, jy)              */
     const float *rw;         /* 8 weights per receiver (4 used in 2-D)       */
+    const awefd_sinc_pt *rsinc;  /* sinc=y: the receivers' taps (else NULL)  */
     float *trow;             /* traces + it*nr (batched: the traces base)     */
     int blk0;                /* batched: first block of this shot             */
     /* batched recompute of a checkpoint segment: c_n of segment k is

@@ -48,6 +48,9 @@ struct awefd_model {
      * 2-D adj=y (image at nt-kt, no traces); inj_rec = the injected points
      * are the receivers (2-D adj=y without expl); nsteps = steps a run takes */
     int sponge, hyb, adj2, inj_rec, nsteps;
+    int sinc;                     /* sinc=y: points on 8-point sinc taps     */
+    awefd_sinc_pt *h_ssinc, *h_rsinc;   /* sinc records of sources/receivers */
+    awefd_sinc_pt *d_rsinc;
     float *d_damp_h;              /* damp table, nb - nop entries (hyb)      */
     float *h_wav;                 /* adj=y: host copy of the wavelet, and    */
     float *d_wavr;                /* its rows back to front on the device,   */

@@ -6,6 +6,7 @@
  * and every expression keeps the reference's float/double types.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -229,4 +230,131 @@ void awefd_sort_inj(awefd_inj_key *keys, long n, int nbucket, awefd_inj *out, in
         ibeg[keys[i].bucket + 1]++;
     }
     for (int b = 0; b < nbucket; b++) ibeg[b + 1] += ibeg[b];
+}
+
+/* ------------------------------------------------------------------ */
+/* 8-point sinc taps of sinc2d_make / sinc3d_make (fdutil.c:784-842,
+ * 1088-1145).  Their weights are the response of ints8r -- the CWP/SU
+ * library's table-driven 8-point least-squares sinc interpolator (intsinc8.c,
+ * mksinc.c, stoep.c), which the reference links from Madagascar -- to a unit
+ * impulse.  The method: for a fractional shift d the 8 taps s solve the
+ * symmetric Toeplitz normal equations
+ *     sum_k sinc(fmax (j - k)) s_k = sinc(fmax (3 - j + d)),
+ *     fmax = min(0.066 + 0.265 ln 8, 1),
+ * by Levinson's recursion in double; shifts are tabulated at k/512 and an
+ * output point uses the row nearest its fractional position. */
+enum { SINC_TAPS = 8, SINC_ROWS = 513 };
+
+static double sinc_norm(double x)
+{
+    if (x == 0.0) return 1.0;
+    const double px = 3.141592653589793 * x;
+    return sin(px) / px;
+}
+
+/* taps of one fractional shift d (Levinson, with the prediction-error
+ * filter e and its power v updated in the order of stoep.c) */
+static void sinc_taps(float d, float *out)
+{
+    double r[SINC_TAPS], g[SINC_TAPS], f[SINC_TAPS], e[SINC_TAPS];
+    double fmax = 0.066 + 0.265 * log((double)SINC_TAPS);
+    fmax = fmax < 1.0 ? fmax : 1.0;
+    for (int j = 0; j < SINC_TAPS; j++) {
+        r[j] = sinc_norm(fmax * j);
+        g[j] = sinc_norm(fmax * (SINC_TAPS / 2 - j - 1 + d));
+    }
+    e[0] = 1.0;
+    double v = r[0];
+    f[0] = g[0] / r[0];
+    for (int j = 1; j < SINC_TAPS; j++) {
+        e[j] = 0.0;
+        f[j] = 0.0;
+        double dot = 0.0;
+        for (int i = 0; i < j; i++) dot += e[i] * r[j - i];
+        double k = dot / v;
+        v -= k * dot;
+        for (int i = 0; i <= j / 2; i++) {
+            const double mirror = e[j - i] - k * e[i];
+            e[i] -= k * e[j - i];
+            e[j - i] = mirror;
+        }
+        double fr = 0.0;
+        for (int i = 0; i < j; i++) fr += f[i] * r[j - i];
+        k = (fr - g[j]) / v;
+        for (int i = 0; i <= j; i++) f[i] -= k * e[j - i];
+    }
+    for (int j = 0; j < SINC_TAPS; j++) out[j] = (float)f[j];
+}
+
+static float sinc_rows[SINC_ROWS][SINC_TAPS];
+static pthread_once_t sinc_once = PTHREAD_ONCE_INIT;
+
+static void sinc_build(void)
+{
+    for (int k = 1; k < SINC_ROWS - 1; k++) sinc_taps((float)k / (float)(SINC_ROWS - 1), sinc_rows[k]);
+    memset(sinc_rows[0], 0, sizeof sinc_rows[0]);
+    memset(sinc_rows[SINC_ROWS - 1], 0, sizeof sinc_rows[0]);
+    sinc_rows[0][SINC_TAPS / 2 - 1] = 1.f;
+    sinc_rows[SINC_ROWS - 1][SINC_TAPS / 2] = 1.f;
+}
+
+/* ints8r(9, 1, -4, impulse at the centre, 0, 0, 9, xo, w): output x uses
+ * the taps from int(12 + x) - 11 on, table row round(frac * 512); the impulse
+ * (input 4) contributes the row's weight at tap 4 - first, when in range. */
+static void sinc_impulse_response(const float *xo, float *w)
+{
+    pthread_once(&sinc_once, sinc_build);
+    const float rows1 = (float)(SINC_ROWS - 1);
+    for (int i = 0; i < 9; i++) {
+        const float xn = 12.0f + xo[i] * 1.0f;
+        const int ixn = (int)xn;
+        const int first = ixn - 11;
+        const float frac = xn - (float)ixn;
+        const int row = frac >= 0.0 ? (int)(frac * rows1 + 0.5) : (int)((frac + 1.0) * rows1 - 0.5);
+        const int t = 4 - first;
+        w[i] = t >= 0 && t < SINC_TAPS ? 1.0f * sinc_rows[row][t] : 0.0f;
+    }
+}
+
+/* one axis: q = (node - point)/h as the reference computes it; the z axis
+ * forms its 9 output positions in double (-4.0 + q + i), x and y in float */
+void awefd_sinc_axis(float q, int zaxis, float w[9], int *f, int *n)
+{
+    float xo[9];
+    for (int i = 0; i < 9; i++)
+        xo[i] = zaxis ? (float)(-4.0 + (double)q + (double)(i * 1.0f)) : -4.0f + q + i * 1.0f;
+    sinc_impulse_response(xo, w);
+    const int on_node = w[4] == 1.0f;
+    *f = on_node ? 4 : 0;
+    *n = on_node ? 1 : 9;
+}
+
+/* Sinc records of na points (cart2d (x,z), cart3d (x,y,z) coordinates) on
+ * a padded grid (z, x, y order).  Returns -1 when a tap of a point leaves
+ * the padded grid (the reference would index outside its arrays). */
+int awefd_sinc_points(int ndim, int na, const float *coords, const int npad[3], const float opad[3],
+                      const float d[3], awefd_sinc_pt *out)
+{
+    for (int a = 0; a < na; a++) {
+        const float *p = coords + (long)ndim * a;
+        awefd_sinc_pt *s = &out[a];
+        const float pos[3] = {p[ndim - 1], p[0], ndim == 3 ? p[1] : 0.f};
+        for (int ax = 0; ax < 3; ax++) {
+            if (ax >= ndim) {           /* 2-D: one tap of weight 1 on y = 0 */
+                s->c[ax] = 0;
+                s->f[ax] = 4;
+                s->n[ax] = 1;
+                for (int i = 0; i < 9; i++) s->w[ax][i] = i == 4 ? 1.f : 0.f;
+                continue;
+            }
+            /* fdutil.c:802-806,815-819,828-832 */
+            const int c = (int)((pos[ax] - opad[ax]) / d[ax] + 0.499f);
+            const float off = c * d[ax] + opad[ax] - pos[ax];
+            s->c[ax] = c;
+            awefd_sinc_axis(off / d[ax], ax == 0, s->w[ax], &s->f[ax], &s->n[ax]);
+            const int lo = c - 4 + s->f[ax], hi = lo + s->n[ax] - 1;
+            if (lo < 0 || hi >= npad[ax]) return -1;
+        }
+    }
+    return 0;
 }

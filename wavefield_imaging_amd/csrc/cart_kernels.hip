@@ -78,12 +78,32 @@ __device__ __forceinline__ bool in_sponge(int iz, int ix, int iy, const awefd_ca
 }
 
 /* lint2d_extract / lint3d_extract of the reference's u0, i.e. the current
- * field with its pending sponge factor. */
+ * field with its pending sponge factor; with sinc taps sinc2d_extract /
+ * sinc3d_extract (fdutil.c:1015-1051,1301-1332): taps y, x, z in order,
+ * g += ((u wy) wx) wz (2-D (u wx) wz) from g = 0. */
 template <int NDIM>
 __device__ void cart_extract(const awefd_cart_args &a, const float *C, float *trow, int r)
 {
     if (r >= a.nr) return;
     const long sx = a.ld, sy = (long)a.ld * a.n2;
+    if (a.rsinc) {
+        const awefd_sinc_pt &p = a.rsinc[r];
+        float g = 0.f;
+        for (int ty = p.f[2]; ty < p.f[2] + p.n[2]; ty++) {
+            const int y = p.c[2] - 4 + ty;
+            for (int tx = p.f[1]; tx < p.f[1] + p.n[1]; tx++) {
+                const int x = p.c[1] - 4 + tx;
+                for (int tz = p.f[0]; tz < p.f[0] + p.n[0]; tz++) {
+                    const int z = p.c[0] - 4 + tz;
+                    float u = C[y * sy + x * sx + z];
+                    if (a.dabc) u = sponge<NDIM>(u, z, x, y, a);
+                    g = NDIM == 3 ? g + u * p.w[2][ty] * p.w[1][tx] * p.w[0][tz] : g + u * p.w[1][tx] * p.w[0][tz];
+                }
+            }
+        }
+        trow[r] = g;
+        return;
+    }
     const int jz = a.rj[3 * r], jx = a.rj[3 * r + 1], jy = a.rj[3 * r + 2];
     constexpr int NC = NDIM == 3 ? 8 : 4;
     float s = 0.f;
@@ -400,7 +420,7 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
                 const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
 #pragma unroll
                 for (int j = 0; j < RX; j++) {
-                    const float inj = nv[j] + amp * en.w * bv_[B][j];
+                    const float inj = nv[j] + amp * en.w * en.w2 * en.w3 * bv_[B][j];
                     nv[j] = (zin && en.z == iz && en.x == x0 + j) ? inj : nv[j];
                 }
             }
@@ -652,7 +672,7 @@ __device__ __forceinline__ void cart_step2_body(const awefd_cart_args &a, int bi
         for (int e = ib; e < ie; e++) {
             const awefd_inj en = a.inj2[e];
             const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
-            const float inj = nv + amp * en.w * vv[j];
+            const float inj = nv + amp * en.w * en.w2 * en.w3 * vv[j];
             nv = (zin && en.z == iz && en.x == x0 + j) ? inj : nv;
         }
         float w = nv * spz;
@@ -685,7 +705,7 @@ __device__ __forceinline__ void cart_step2_body(const awefd_cart_args &a, int bi
         for (int e = ib; e < ie; e++) {
             const awefd_inj en = a.inj2[e];
             const float amp = a.wav[(long)(it + 1) * a.wstride + (a.wexpl ? 0 : en.src)];
-            const float inj = nv + amp * en.w * vv[j];
+            const float inj = nv + amp * en.w * en.w2 * en.w3 * vv[j];
             nv = (zin && en.z == iz && en.x == x0 + j) ? inj : nv;
         }
         float w = nv * spz;
