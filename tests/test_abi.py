@@ -37,3 +37,56 @@ def test_params_default_matches_reference():
         assert p.dabc == 0 and p.fsrf == 0
     assert lib.awefd_params_default(7, C.byref(L.Params())) != 0
     assert b"bad kind" in lib.awefd_last_error()
+
+
+def test_params_layout_matches_the_header(tmp_path):
+    """The ctypes mirror of awefd_params has the C struct's size and field
+    offsets (compiled from include/awefd.h with the host compiler)."""
+    import subprocess
+
+    fields = [f for f, _ in L.Params._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "awefd.h"\nint main(void){\n'
+                   '  printf("size %zu\\n", sizeof(awefd_params));\n'
+                   + "".join('  printf("%s %%zu\\n", offsetof(awefd_params, %s));\n' % (f, f) for f in fields)
+                   + "  return 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                       text=True).stdout.splitlines())
+    assert int(out["size"]) == C.sizeof(L.Params)
+    for f in fields:
+        assert int(out[f]) == getattr(L.Params, f).offset, f
+
+
+def test_create_rejects_an_unreachable_image_step():
+    """2-D adj=y images at step nt-kt: kt outside 1..nt is refused before any
+    device call."""
+    lib = L.load()
+    for kt in (0, -3, 41):
+        p = L.Params()
+        lib.awefd_params_default(L.CART2D, C.byref(p))
+        p.n[0], p.n[1], p.nt, p.dt = 20, 30, 40, 1e-8
+        p.adj, p.kt = 1, kt
+        h = C.c_void_p()
+        assert lib.awefd_create(C.byref(p), C.byref(h)) == -1 and not h.value
+        assert b"kt" in lib.awefd_last_error()
+
+
+def test_sinc_weights_equal_the_oracle(orc):
+    """The library's host-side sinc taps (sinc=y) equal the oracle's bit for
+    bit, on and off the nodes, for the float (x, y) and double (z) offset
+    forms."""
+    import numpy as np
+
+    lib = L.load()
+    f = C.POINTER(C.c_float)
+    rng = np.random.default_rng(8)
+    qs = np.concatenate([rng.uniform(-0.5, 0.5, 3000), [0.0, 0.5, -0.5, 0.25, 1e-7, -1e-7]]).astype(np.float32)
+    for q in qs:
+        for zaxis in (0, 1):
+            a, b = np.zeros(9, np.float32), np.zeros(9, np.float32)
+            fa, na, fb, nb = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+            lib.awefd_sinc_axis(C.c_float(q), zaxis, a.ctypes.data_as(f), C.byref(fa), C.byref(na))
+            orc.lib().orc_sinc_axis(C.c_float(q), zaxis, b.ctypes.data_as(f), C.byref(fb), C.byref(nb))
+            assert a.tobytes() == b.tobytes() and (fa.value, na.value) == (fb.value, nb.value), (q, zaxis)
