@@ -185,14 +185,14 @@ int awefd_batch_tune(awefd_batch *b, int steps, int *rx_out, int *group_mb_out, 
  * awefd_get_traces, the field via awefd_get_field. */
 int awefd_batch_run(awefd_batch *b);
 void awefd_batch_destroy(awefd_batch *b);
-/* Steps per launch of awefd_batch_run: 1 (default) or 2 (tile rx 4 or 8).
- * With 2, each launch advances every shot two steps on overlapping tiles
- * (the first step's field kept in LDS), reading the fields once per two
- * steps; results are bit-identical. */
+/* Steps per launch of awefd_batch_run: 1 (default), 2 or 3 (tile rx 4 or
+ * 8).  With 2 or 3, each launch advances every shot that many steps on
+ * overlapping tiles (each step's field kept in LDS for the next), reading
+ * the fields once per launch; results are bit-identical. */
 int awefd_batch_set_depth(awefd_batch *b, int depth);
-/* awefd_batch_tune over the depth as well (depth 1: rx 2, 4, 8, 16; depth 2:
- * rx 4, 8) and keep the fastest; ms_out[2*4*5] holds ms per step, depth-major
- * then row-major by rx (0 for the choices not tried). */
+/* awefd_batch_tune over the depth as well (depth 1: rx 2, 4, 8, 16; depths
+ * 2 and 3: rx 4, 8) and keep the fastest; ms_out[3*4*5] holds ms per step,
+ * depth-major then row-major by rx (0 for the choices not tried). */
 int awefd_batch_tune_depth(awefd_batch *b, int steps, int *rx_out, int *group_mb_out, int *depth_out,
                            double *ms_out);
 /* The batch's awefd_run_ckpt: every shot's forward loop with checkpoints

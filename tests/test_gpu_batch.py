@@ -112,9 +112,10 @@ def test_batch_scalar_order_and_free_surface(orc):
     (58, 6, dict(free=True)), (55, 4, dict(optfd=True)),
 ])
 def test_two_steps_per_pass_match_single_runs(orc, nt, fdorder, kw):
-    """Depth 2 (two time steps per launch on overlapping tiles, the first
-    step's field through LDS): every shot's traces and final field equal its
-    own single-step run bit for bit, for odd and even nt, every stencil
+    """Depth 2 and 3 (two or three time steps per launch on overlapping
+    tiles, each step's field through LDS; with three, the middle trace row
+    extracted in-block): every shot's traces and final field equal its own
+    single-step run bit for bit, for nt of every residue, every stencil
     order, both operation orders, one and several shot groups, and on graph
     replays."""
     shots = _shots(6, nt, seed=11, fdorder=fdorder, **kw)
@@ -122,14 +123,15 @@ def test_two_steps_per_pass_match_single_runs(orc, nt, fdorder, kw):
     ref = _single(shots)
     ms = _models(shots)
     b = awefd.Batch(ms)
-    for rx, grp in ((4, 0), (8, 1), (4, 1)):
+    for rx, grp, dep in ((4, 0, 2), (8, 1, 2), (4, 1, 2), (4, 0, 3), (8, 1, 3), (4, 1, 3)):
         b.set_tile(rx, grp)
-        b.set_depth(2)
+        b.set_depth(dep)
         for rep in range(2):
             b.run()
             for s, m in enumerate(ms):
-                np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg="shot %d rx %d rep %d" % (s, rx, rep))
-                np.testing.assert_array_equal(m.field(), ref[s][1], err_msg="shot %d rx %d rep %d" % (s, rx, rep))
+                msg = "shot %d rx %d depth %d rep %d" % (s, rx, dep, rep)
+                np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg=msg)
+                np.testing.assert_array_equal(m.field(), ref[s][1], err_msg=msg)
     tr, fin, _ = orc.run("cart2d", *shots[0])
     np.testing.assert_array_equal(ref[0][0], tr)
     b.close()
@@ -151,14 +153,15 @@ def test_two_steps_sources_on_tile_edges_and_tuning(orc):
     ref = _single(out)
     ms = _models(out)
     b = awefd.Batch(ms)
-    b.set_tile(4, 0)
-    b.set_depth(2)
-    b.run()
-    for s, m in enumerate(ms):
-        np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg="shot %d" % s)
+    for dep in (2, 3):
+        b.set_tile(4, 0)
+        b.set_depth(dep)
+        b.run()
+        for s, m in enumerate(ms):
+            np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg="shot %d depth %d" % (s, dep))
     rx, grp, depth, times = b.tune_depth(steps=10)
     print("tuned rx", rx, "group", grp, "depth", depth, flush=True)
-    assert depth in (1, 2) and all(t > 0 for t in times.values())
+    assert depth in (1, 2, 3) and all(t > 0 for t in times.values())
     b.run()
     for s, m in enumerate(ms):
         np.testing.assert_array_equal(m.traces(), ref[s][0], err_msg="tuned shot %d" % s)

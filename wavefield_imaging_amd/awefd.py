@@ -225,7 +225,7 @@ class Batch:
         L.check(self.lib.awefd_batch_run(self.h))
 
     def set_depth(self, depth):
-        """Steps per launch: 1, or 2 (tile rx 4 or 8)."""
+        """Steps per launch: 1, or 2 or 3 (tile rx 4 or 8)."""
         L.check(self.lib.awefd_batch_set_depth(self.h, int(depth)))
 
     def tune_depth(self, steps=20):
@@ -233,9 +233,9 @@ class Batch:
         returns (rx, group_mb, depth, {(depth, rx, group_mb): ms per step})."""
         rx, grp, dep = C.c_int(), C.c_int(), C.c_int()
         n = len(self.RX) * len(self.GROUP_MB)
-        ms = (C.c_double * (2 * n))()
+        ms = (C.c_double * (3 * n))()
         L.check(self.lib.awefd_batch_tune_depth(self.h, int(steps), C.byref(rx), C.byref(grp), C.byref(dep), ms))
-        keys = [(d, r, g) for d in (1, 2) for r in self.RX for g in self.GROUP_MB]
+        keys = [(d, r, g) for d in (1, 2, 3) for r in self.RX for g in self.GROUP_MB]
         return rx.value, grp.value, dep.value, {k: v for k, v in zip(keys, list(ms)) if v > 0}
 
     def run_ckpt(self, ckpt=0):

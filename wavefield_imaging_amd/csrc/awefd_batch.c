@@ -72,19 +72,21 @@ struct awefd_batch {
     long *d_zcbeg, znch;
     long abshot_cap;
     float **fwd_ck;                  /* checkpoint bases the forward graph holds */
-    /* two time steps per pass (depth 2) */
-    int depth;                       /* 1 or 2                                   */
+    /* two or three time steps per pass (depth 2, 3) */
+    int depth;                       /* 1, 2 or 3                                */
     float **e0, **e1;                /* per shot: the extra field pair           */
     int *d_bshot2;                   /* two-step launch: block -> shot           */
     long *goff2;
     int *gnblk2;
     awefd_inj **d_inj2;              /* per shot: two-step injection buckets     */
     int **d_ibeg2;
-    int t2_ok;                       /* two-step tables built for the tile       */
+    int **d_rbeg3, **d_rlist3;       /* per shot: three-step receiver buckets    */
+    int t2_ok;                       /* multi-step tables built for the tile     */
+    int t2_depth;                    /* ... and for this depth                   */
     awefd_copy_job *d_fjobs;         /* end of a depth-2 run: fields back into   */
     long *d_fcbeg, fnch;             /* the models' own buffers                  */
     int nfjob, fin_p, fin_c;
-    double tune2_ms[4][NGC];
+    double tune2_ms[4][NGC], tune3_ms[4][NGC];
     hipGraph_t gfwd, gbwd;
     hipGraphExec_t xfwd, xbwd;
     int fwd_ok, bwd_ok;
@@ -127,11 +129,15 @@ void awefd_batch_destroy(awefd_batch *b)
         if (b->e1) dfree(&b->e1[s]);
         if (b->d_inj2) dfree(&b->d_inj2[s]);
         if (b->d_ibeg2) dfree(&b->d_ibeg2[s]);
+        if (b->d_rbeg3) dfree(&b->d_rbeg3[s]);
+        if (b->d_rlist3) dfree(&b->d_rlist3[s]);
     }
     free(b->e0);
     free(b->e1);
     free(b->d_inj2);
     free(b->d_ibeg2);
+    free(b->d_rbeg3);
+    free(b->d_rlist3);
     dfree(&b->d_bshot2);
     dfree(&b->d_fjobs);
     dfree(&b->d_fcbeg);
@@ -313,7 +319,7 @@ static int batch_build(awefd_batch *b, int rx, int group_mb)
     /* captured graphs refer to the tables just replaced */
     b->graph_ok = b->fwd_ok = b->bwd_ok = 0;
     b->t2_ok = 0;
-    if (!rc && b->depth == 2 && rx != 4 && rx != 8) b->depth = 1;
+    if (!rc && b->depth >= 2 && rx != 4 && rx != 8) b->depth = 1;
     return rc;
 }
 
@@ -353,14 +359,25 @@ static int t2_tile(const awefd_batch *b, int *sz, int *sx)
     return (b->rx == 4 || b->rx == 8) && *sx > 0;
 }
 
+/* Tile geometry of a multi-step pass: depth 2 as above; depth 3 blocks of
+ * 8 waves (64 z x 8 rx rows) with outputs 2 nop inside on every side. */
+static int tn_tile(const awefd_batch *b, int depth, int *sz, int *sx, int *rows, int *halo)
+{
+    const int nw = depth == 3 ? AWEFD_BATCH_NWAVE3 : AWEFD_BATCH_NWAVE;
+    *halo = (depth - 1) * b->nop;
+    *rows = nw * b->rx;
+    *sz = 64 - 2 * *halo;
+    *sx = *rows - 2 * *halo;
+    return (depth == 2 || depth == 3) && (b->rx == 4 || b->rx == 8) && *sz > 0 && *sx > 0;
+}
+
 /* every (source node, tile) pair whose tile's first-step region holds the
  * node, bucketed per tile */
-static int t2_buckets(const awefd_batch *b, const awefd_model *m, int ntz2, int ntx2, awefd_inj **inj, int **ibeg,
-                      long *ninj)
+static int t2_buckets(const awefd_batch *b, const awefd_model *m, int depth, int ntz2, int ntx2, awefd_inj **inj,
+                      int **ibeg, long *ninj)
 {
-    int sz, sx;
-    t2_tile(b, &sz, &sx);
-    const int rows = AWEFD_BATCH_NWAVE * b->rx, nop = b->nop;
+    int sz, sx, rows, nop;
+    tn_tile(b, depth, &sz, &sx, &rows, &nop);   /* nop: the tile halo */
     long cap = 16L * 4 * (m->ns > 0 ? m->ns : 1), n = 0;
     awefd_inj_key *keys = malloc(sizeof(*keys) * cap);
     int *beg = malloc(sizeof(int) * (ntz2 * ntx2 + 1));
@@ -410,19 +427,46 @@ static int t2_buckets(const awefd_batch *b, const awefd_model *m, int ntz2, int 
 }
 
 /* Two-step tables for the current tile and groups. */
+/* receivers bucketed by the tile whose outputs hold their first corner */
+static int t3_receivers(const awefd_model *m, int sz, int sx, int ntz, int ntx, int **rbeg, int **rlist)
+{
+    const int nt = ntz * ntx, nr = m->nr;
+    int *beg = calloc(nt + 1, sizeof(int)), *list = malloc(sizeof(int) * (nr > 0 ? nr : 1));
+    if (!beg || !list) {
+        free(beg); free(list);
+        return awefd__fail(AWEFD_ENOMEM, "three-step receiver buckets");
+    }
+    for (int r = 0; r < nr; r++) beg[m->h_rj[3 * r] / sz + ntz * (m->h_rj[3 * r + 1] / sx) + 1]++;
+    for (int t = 0; t < nt; t++) beg[t + 1] += beg[t];
+    int *fill = malloc(sizeof(int) * (nt > 0 ? nt : 1));
+    if (!fill) {
+        free(beg); free(list);
+        return awefd__fail(AWEFD_ENOMEM, "three-step receiver buckets");
+    }
+    memcpy(fill, beg, sizeof(int) * nt);
+    for (int r = 0; r < nr; r++) list[fill[m->h_rj[3 * r] / sz + ntz * (m->h_rj[3 * r + 1] / sx)]++] = r;
+    free(fill);
+    *rbeg = beg;
+    *rlist = list;
+    return AWEFD_OK;
+}
+
 static int t2_build(awefd_batch *b)
 {
-    int sz, sx;
-    if (!t2_tile(b, &sz, &sx))
-        return awefd__fail(AWEFD_EINVAL, "two steps per pass need tile rx 4 or 8 (and 4 rx > 2 nop)");
-    const int nthr = AWEFD_BATCH_NWAVE * 64, ns = b->nshot, ng = b->ngroup;
+    int sz, sx, rows, halo;
+    if (!tn_tile(b, b->depth, &sz, &sx, &rows, &halo))
+        return awefd__fail(AWEFD_EINVAL, "%d steps per pass need tile rx 4 or 8 (and a tile wider than its halo)",
+                           b->depth);
+    const int nthr = (b->depth == 3 ? AWEFD_BATCH_NWAVE3 : AWEFD_BATCH_NWAVE) * 64, ns = b->nshot, ng = b->ngroup;
     if (!b->e0) b->e0 = calloc(ns, sizeof(float *));
     if (!b->e1) b->e1 = calloc(ns, sizeof(float *));
     if (!b->d_inj2) b->d_inj2 = calloc(ns, sizeof(awefd_inj *));
     if (!b->d_ibeg2) b->d_ibeg2 = calloc(ns, sizeof(int *));
+    if (!b->d_rbeg3) b->d_rbeg3 = calloc(ns, sizeof(int *));
+    if (!b->d_rlist3) b->d_rlist3 = calloc(ns, sizeof(int *));
     long *goff2 = calloc(ng + 1, sizeof(long));
     int *gnblk2 = calloc(ng, sizeof(int)), *nb = malloc(sizeof(int) * ns);
-    int rc = (!b->e0 || !b->e1 || !b->d_inj2 || !b->d_ibeg2 || !goff2 || !gnblk2 || !nb)
+    int rc = (!b->e0 || !b->e1 || !b->d_inj2 || !b->d_ibeg2 || !b->d_rbeg3 || !b->d_rlist3 || !goff2 || !gnblk2 || !nb)
                  ? awefd__fail(AWEFD_ENOMEM, "two-step tables") : AWEFD_OK;
     for (int s = 0; s < ns && !rc; s++) {
         awefd_model *m = b->m[s];
@@ -448,7 +492,7 @@ static int t2_build(awefd_batch *b)
         awefd_inj *inj = NULL;
         int *ibeg = NULL;
         long ninj = 0;
-        if (!rc) rc = t2_buckets(b, m, h->ntz2, h->ntx2, &inj, &ibeg, &ninj);
+        if (!rc) rc = t2_buckets(b, m, b->depth, h->ntz2, h->ntx2, &inj, &ibeg, &ninj);
         if (!rc) {
             dfree(&b->d_inj2[s]);
             dfree(&b->d_ibeg2[s]);
@@ -462,6 +506,26 @@ static int t2_build(awefd_batch *b)
         }
         free(inj);
         free(ibeg);
+        h->rbeg3 = NULL;
+        h->rlist3 = NULL;
+        if (!rc && b->depth == 3) {
+            int *rbeg = NULL, *rlist = NULL;
+            rc = t3_receivers(m, sz, sx, h->ntz2, h->ntx2, &rbeg, &rlist);
+            if (!rc) {
+                dfree(&b->d_rbeg3[s]);
+                dfree(&b->d_rlist3[s]);
+                const size_t nbeg = (size_t)h->ntz2 * h->ntx2 + 1, nrl = m->nr > 0 ? m->nr : 1;
+                e = hipMalloc((void **)&b->d_rbeg3[s], sizeof(int) * nbeg);
+                if (e == hipSuccess) e = hipMalloc((void **)&b->d_rlist3[s], sizeof(int) * nrl);
+                if (e == hipSuccess) e = hipMemcpy(b->d_rbeg3[s], rbeg, sizeof(int) * nbeg, hipMemcpyHostToDevice);
+                if (e == hipSuccess) e = hipMemcpy(b->d_rlist3[s], rlist, sizeof(int) * nrl, hipMemcpyHostToDevice);
+                if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "three-step receiver upload: %s", hipGetErrorString(e));
+                h->rbeg3 = b->d_rbeg3[s];
+                h->rlist3 = b->d_rlist3[s];
+            }
+            free(rbeg);
+            free(rlist);
+        }
     }
     for (int g = 0; g < ng && !rc; g++) goff2[g + 1] = goff2[g] + gnblk2[g];
     int *bshot = rc ? NULL : malloc(sizeof(int) * (goff2[ng] > 0 ? goff2[ng] : 1));
@@ -483,6 +547,7 @@ static int t2_build(awefd_batch *b)
         b->goff2 = goff2;
         b->gnblk2 = gnblk2;
         b->t2_ok = 1;
+        b->t2_depth = b->depth;
     } else {
         free(goff2);
         free(gnblk2);
@@ -494,18 +559,27 @@ static int t2_build(awefd_batch *b)
  * steps (each pass extracts the rows of its two input fields, so ending on
  * single steps extracts every row); final roles of the previous and current
  * field in *fp, *fc */
-static int t2_steps(awefd_batch *b, int g, int nsteps, int *fp, int *fc)
+/* Passes of `depth` steps while at least one more step follows, then
+ * single steps: a multi-step pass at `it` extracts rows it-1 and it (and,
+ * with three steps, it+1 in-block), so every row is extracted.  With launch
+ * set the passes of group g are issued; the final roles of the previous and
+ * current field go to *fp, *fc. */
+static int tn_schedule(awefd_batch *b, int depth, int g, int nsteps, int *fp, int *fc, int launch)
 {
     int iP = 0, iC = 1, it = 0;
-    const int npair = (nsteps - 1) / 2;
-    for (int k = 0; k < npair; k++, it += 2) {
+    const int nmulti = (nsteps - 1) / depth;
+    for (int k = 0; k < nmulti; k++, it += depth) {
         int d[2], nd = 0;
         for (int i = 0; i < 4; i++)
             if (i != iP && i != iC) d[nd++] = i;
         const int roles = iP | iC << 2 | d[0] << 4 | d[1] << 6;
-        int e = awefd_launch_cart_batch_t2(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it, roles, 1, b->nop,
-                                           b->sse, b->rx, b->stream);
-        if (e) return awefd__fail(AWEFD_EHIP, "two-step pass %d: %s", it, hipGetErrorString((hipError_t)e));
+        if (launch) {
+            int e = depth == 3 ? awefd_launch_cart_batch_t3(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it,
+                                                            roles, 3, b->nop, b->sse, b->rx, b->stream)
+                               : awefd_launch_cart_batch_t2(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it,
+                                                            roles, 1, b->nop, b->sse, b->rx, b->stream);
+            if (e) return awefd__fail(AWEFD_EHIP, "%d-step pass %d: %s", depth, it, hipGetErrorString((hipError_t)e));
+        }
         iP = d[0];
         iC = d[1];
     }
@@ -513,15 +587,26 @@ static int t2_steps(awefd_batch *b, int g, int nsteps, int *fp, int *fc)
         int d0 = 0;
         while (d0 == iP || d0 == iC) d0++;
         const int roles = iP | iC << 2 | d0 << 4 | d0 << 6;
-        int e = awefd_launch_cart_batch_t2(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it, roles, 0, b->nop,
-                                           b->sse, b->rx, b->stream);
-        if (e) return awefd__fail(AWEFD_EHIP, "one-step pass %d: %s", it, hipGetErrorString((hipError_t)e));
+        if (launch) {
+            int e = depth == 3 ? awefd_launch_cart_batch_t3(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it,
+                                                            roles, 1, b->nop, b->sse, b->rx, b->stream)
+                               : awefd_launch_cart_batch_t2(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it,
+                                                            roles, 0, b->nop, b->sse, b->rx, b->stream);
+            if (e) return awefd__fail(AWEFD_EHIP, "one-step pass %d: %s", it, hipGetErrorString((hipError_t)e));
+        }
         iP = iC;
         iC = d0;
     }
     *fp = iP;
     *fc = iC;
     return AWEFD_OK;
+}
+
+/* nsteps steps of group g in multi-step passes (the depth the tables were
+ * built for); final roles of the previous and current field in *fp, *fc */
+static int t2_steps(awefd_batch *b, int g, int nsteps, int *fp, int *fc)
+{
+    return tn_schedule(b, b->t2_depth, g, nsteps, fp, fc, 1);
 }
 
 /* After a depth-2 run the last two fields may sit in the batch's extra pair:
@@ -575,13 +660,13 @@ static int t2_final_jobs(awefd_batch *b, int fp, int fc)
 int awefd_batch_set_depth(awefd_batch *b, int depth)
 {
     if (!b) return awefd__fail(AWEFD_EINVAL, "NULL batch");
-    if (depth != 1 && depth != 2) return awefd__fail(AWEFD_EINVAL, "batch depth %d (1 or 2)", depth);
+    if (depth < 1 || depth > 3) return awefd__fail(AWEFD_EINVAL, "batch depth %d (1, 2 or 3)", depth);
     HIPCHK(hipSetDevice(b->dev));
     int rc;
     if (!b->rx && (rc = batch_build(b, 8, 0))) return rc;
-    int sz, sx;
-    if (depth == 2 && !t2_tile(b, &sz, &sx))
-        return awefd__fail(AWEFD_EINVAL, "two steps per pass need tile rx 4 or 8 (tile is %d)", b->rx);
+    int sz, sx, rows, halo;
+    if (depth >= 2 && !tn_tile(b, depth, &sz, &sx, &rows, &halo))
+        return awefd__fail(AWEFD_EINVAL, "%d steps per pass need tile rx 4 or 8 (tile is %d)", depth, b->rx);
     b->depth = depth;
     b->graph_ok = 0;
     return AWEFD_OK;
@@ -644,13 +729,14 @@ static int time_choice(awefd_batch *b, int depth, int rx, int group_mb, int step
                        double *ms_out)
 {
     int rc = batch_build(b, rx, group_mb), fp, fc;
-    if (!rc && depth == 2) rc = t2_build(b);
+    b->depth = depth;
+    if (!rc && depth >= 2) rc = t2_build(b);
     if (!rc) rc = batch_reset(b);
     /* warm the tables */
-    for (int g = 0; g < b->ngroup && !rc && depth == 2; g++) rc = t2_steps(b, g, 1, &fp, &fc);
+    for (int g = 0; g < b->ngroup && !rc && depth >= 2; g++) rc = t2_steps(b, g, 1, &fp, &fc);
     if (!rc && depth == 1) rc = batch_steps(b, 1);
     if (!rc) rc = (int)hipEventRecord(e0, b->stream);
-    for (int g = 0; g < b->ngroup && !rc && depth == 2; g++) rc = t2_steps(b, g, steps, &fp, &fc);
+    for (int g = 0; g < b->ngroup && !rc && depth >= 2; g++) rc = t2_steps(b, g, steps, &fp, &fc);
     if (!rc && depth == 1) rc = batch_steps(b, steps);
     if (!rc) rc = (int)hipEventRecord(e1, b->stream);
     if (!rc) rc = (int)hipEventSynchronize(e1);
@@ -674,15 +760,16 @@ int awefd_batch_tune_depth(awefd_batch *b, int steps, int *rx_out, int *group_ou
     double best = 0;
     memset(b->tune_ms, 0, sizeof b->tune_ms);
     memset(b->tune2_ms, 0, sizeof b->tune2_ms);
-    for (int d = 1; d <= 2 && !rc; d++)
+    memset(b->tune3_ms, 0, sizeof b->tune3_ms);
+    for (int d = 1; d <= 3 && !rc; d++)
         for (int i = 0; i < 4 && !rc; i++) {
-            int sz, sx;
+            int sz, sx, rows, halo;
             b->rx = RX_CAND[i];
-            if (d == 2 && !t2_tile(b, &sz, &sx)) continue;
+            if (d >= 2 && !tn_tile(b, d, &sz, &sx, &rows, &halo)) continue;
             for (int g = 0; g < NGC && !rc; g++) {
                 double ms;
                 if ((rc = time_choice(b, d, RX_CAND[i], GROUP_CAND[g], steps, e0, e1, &ms))) break;
-                (d == 1 ? b->tune_ms : b->tune2_ms)[i][g] = ms;
+                (d == 1 ? b->tune_ms : d == 2 ? b->tune2_ms : b->tune3_ms)[i][g] = ms;
                 if (bi < 0 || ms < best) {
                     bd = d;
                     bi = i;
@@ -702,6 +789,7 @@ int awefd_batch_tune_depth(awefd_batch *b, int steps, int *rx_out, int *group_ou
             for (int g = 0; g < NGC; g++) {
                 ms_out[i * NGC + g] = b->tune_ms[i][g];
                 ms_out[4 * NGC + i * NGC + g] = b->tune2_ms[i][g];
+                ms_out[8 * NGC + i * NGC + g] = b->tune3_ms[i][g];
             }
     if ((rc = batch_build(b, RX_CAND[bi], GROUP_CAND[bg]))) return rc;
     b->depth = bd;
@@ -718,29 +806,16 @@ static int batch_capture(awefd_batch *b)
     b->gexec = NULL;
     b->graph = NULL;
     int fp = 0, fc = 1, rc;
-    if (b->depth == 2) {
-        if (!b->t2_ok && (rc = t2_build(b))) return rc;
+    if (b->depth >= 2) {
+        if ((!b->t2_ok || b->t2_depth != b->depth) && (rc = t2_build(b))) return rc;
         /* the final roles are those of any group's schedule */
         int i = 0, c = 1;
-        const int npair = (b->nt - 1) / 2;
-        for (int k = 0; k < npair; k++) {
-            int d[2], nd = 0;
-            for (int q = 0; q < 4; q++)
-                if (q != i && q != c) d[nd++] = q;
-            i = d[0];
-            c = d[1];
-        }
-        for (int it = 2 * npair; it < b->nt; it++) {
-            int d0 = 0;
-            while (d0 == i || d0 == c) d0++;
-            i = c;
-            c = d0;
-        }
+        tn_schedule(b, b->depth, 0, b->nt, &i, &c, 0);
         if ((rc = t2_final_jobs(b, i, c))) return rc;
     }
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     rc = batch_reset(b);
-    if (b->depth == 2) {
+    if (b->depth >= 2) {
         for (int g = 0; g < b->ngroup && !rc; g++) rc = t2_steps(b, g, b->nt, &fp, &fc);
         if (!rc && b->nfjob) {
             int e = awefd_launch_copy_jobs(b->d_fjobs, b->d_fcbeg, b->nfjob, b->fnch, b->stream);
@@ -776,7 +851,7 @@ int awefd_batch_run(awefd_batch *b)
      * step (u0 after its swap) is buffer u1 for odd nt, u0 for even nt */
     for (int s = 0; s < b->nshot; s++) {
         awefd_model *m = b->m[s];
-        if (b->depth == 2) {
+        if (b->depth >= 2) {
             m->dP = b->fin_p ? b->u1[s] : b->u0[s];
             m->dC = b->fin_c ? b->u1[s] : b->u0[s];
         } else {

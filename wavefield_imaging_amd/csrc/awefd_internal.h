@@ -21,6 +21,7 @@ extern "C" {
 #endif
 #define AWEFD_CART_TX (AWEFD_NWAVE * AWEFD_RX) /* x rows per stencil block     */
 #define AWEFD_BATCH_NWAVE 4      /* waves per block of the batched 2-D kernel     */
+#define AWEFD_BATCH_NWAVE3 8     /* waves per block of the three-step 2-D pass    */
 
 /* One source-injection contribution: node (z, x, y) += amp(src) * w * vel. */
 typedef struct {
@@ -93,6 +94,9 @@ typedef struct {
     int ntz2, ntx2, nblk2, blk02;
     const int *ibeg2;
     const awefd_inj *inj2;
+    /* batched three-step pass: receivers per tile, by the tile owning their
+     * first interpolation corner (trace row it + 1 is extracted in-block) */
+    const int *rbeg3, *rlist3;
 } awefd_cart_args;
 
 /* Adjoint of the cartesian 2-D loop (adj_kernels.hip). */
@@ -215,6 +219,11 @@ int awefd_launch_fill(float *p, long n, float v, void *stream);
  * the outputs (bits 0-1, 2-3, 4-5, 6-7); extraction blocks write trace rows
  * it (current field) and it-1 (previous field, it > 0). */
 int awefd_launch_cart_batch_t2(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int two,
+                               int nop, int sse, int rx, void *stream);
+/* Batched 2-D steps it, it+1, it+2 of every shot in one pass (nst = 3), or
+ * step it alone (nst = 1) on the same tiles: roles as above (outputs: the
+ * second and third steps' fields); rx in {4, 8} with 8 waves per block. */
+int awefd_launch_cart_batch_t3(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int nst,
                                int nop, int sse, int rx, void *stream);
 /* recompute step n of every shot inside its checkpoint segment (out of
  * place, no extraction), same tables as awefd_launch_cart_batch2 */

@@ -725,6 +725,220 @@ __global__ __launch_bounds__(NW * WAVE) void k_cart_batch_t2(const awefd_cart_ar
     cart_step2_body<NOP, SSE, RX, NW>(a, (int)blockIdx.x - a.blk02, it, roles, two);
 }
 
+/* Three steps per pass (batched 2-D).  A block covers 64 z x NW*RX x rows
+ * and computes the first step on all of them, the second on the nodes at
+ * least nop inside, the third on those at least 2 nop inside: outputs are
+ * (64 - 4 nop) x (NW*RX - 4 nop) nodes.  Each step's field passes to the
+ * next through LDS.  The pass reads the previous and current field and V
+ * once and writes the second and third steps' fields (D1, D2); the first
+ * step's trace row, it + 1, is extracted in the block from LDS by the tile
+ * that owns the receiver's first corner (rbeg3/rlist3), the others by the
+ * following passes' extraction blocks.  nst = 1 takes one step (writing D1)
+ * with the same tiles, for the end of the loop. */
+template <int NOP, bool SSE, int RX, int NW>
+__device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bid, int it, int roles, int nst)
+{
+    constexpr int ROWS = NW * RX;
+    constexpr int H = 2 * NOP;                    /* tile halo for three steps */
+    constexpr int SZ = WAVE - 2 * H;
+    constexpr int SX = ROWS - 2 * H;
+    static_assert(SX > 0 && SZ > 0, "three-step tile too small for the stencil");
+    const float *Pin = a.F[roles & 3], *Cin = a.F[(roles >> 2) & 3];
+    float *D1 = a.F[(roles >> 4) & 3], *D2 = a.F[(roles >> 6) & 3];
+    const int tid = threadIdx.x;
+    if (bid >= a.nblk2) {
+        /* trace rows it (current field) and it-1 (previous field) */
+        const int r = (bid - a.nblk2) * (NW * WAVE) + tid;
+        cart_extract<2>(a, Cin, a.trow + (long)it * a.nr, r);
+        if (it > 0) cart_extract<2>(a, Pin, a.trow + (long)(it - 1) * a.nr, r);
+        return;
+    }
+    const int lane = tid & (WAVE - 1);
+    const int wv = __builtin_amdgcn_readfirstlane(tid / WAVE);
+    const int lb = xcd_remap(bid, a.nblk2);
+    const int tzi = lb % a.ntz2, txi = lb / a.ntz2;
+    const int n1 = a.n1, n2 = a.n2;
+    const int z0 = tzi * SZ - H;                  /* z of lane 0           */
+    const int iz = z0 + lane;
+    const bool zin = iz >= 0 && iz < n1;
+    const int bx0 = txi * SX - H;                 /* the block's first row */
+    const int x0 = bx0 + wv * RX;                 /* wave-uniform          */
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    const unsigned nbytes = (unsigned)a.ld * (unsigned)n2 * 4u;
+    const rsrc_t rC = mk_rsrc(Cin, nbytes), rP = mk_rsrc(Pin, nbytes), rV = mk_rsrc(a.vel, nbytes);
+    const rsrc_t rD1 = mk_rsrc(D1, nbytes), rD2 = mk_rsrc(D2, nbytes);
+    const unsigned vz = zin ? (unsigned)iz * 4u : OOB;
+    int zz = -1;
+    if (lane < NOP) zz = z0 - 1 - lane;
+    else if (lane >= WAVE - NOP) zz = z0 + WAVE + (WAVE - 1 - lane);
+    const unsigned vzh = (zz >= 0 && zz < n1) ? (unsigned)zz * 4u : OOB;
+    const bool zint = iz >= NOP && iz < n1 - NOP;
+    const bool zsp = a.dabc && zin && (iz < a.nb || iz >= n1 - a.nb);
+    const float spz = zsp ? a.sp1[iz] : 1.f;
+    /* this lane's node is an output of the tile */
+    const unsigned vzo = (lane >= H && lane < WAVE - H) ? vz : OOB;
+
+    float c[RX], p[RX], vv[RX], zh[RX], spx[RX];
+    unsigned xint_m = 0;
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const int ix = x0 + j;
+        const unsigned so = (unsigned)ix * rowb;  /* rows outside the grid: beyond the range */
+        c[j] = bld(rC, vz, so);
+        p[j] = bld<NT>(rP, vz, so);
+        vv[j] = bld<NT>(rV, vz, so);
+        zh[j] = bld(rC, vzh, so);
+        const bool xsp = a.dabc && ix >= 0 && ix < n2 && (ix < a.nb || ix >= n2 - a.nb);
+        spx[j] = xsp ? a.sp2[ix] : 1.f;
+        xint_m |= (ix >= NOP && ix < n2 - NOP) ? 1u << j : 0u;
+    }
+    float hg[2 * NOP];
+#pragma unroll
+    for (int h = 0; h < 2 * NOP; h++) {
+        const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
+        const bool outside = hx < bx0 || hx >= bx0 + ROWS;
+        hg[h] = bld(rC, outside ? vz : OOB, (unsigned)hx * rowb);
+    }
+    const int col = tzi + a.ntz2 * txi;
+    const int ib = a.ibeg2[col], ie = a.ibeg2[col + 1];
+
+    __shared__ float xr[3][ROWS][WAVE];
+#pragma unroll
+    for (int j = 0; j < RX; j++) xr[0][wv * RX + j][lane] = c[j];
+    __syncthreads();
+
+    /* first step (it) on every lane and row */
+    float s1[RX];
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        float zm[NOP], zp[NOP], xm[NOP], xp[NOP];
+        zneigh<NOP>(c[j], zh[j], zm, zp);
+#pragma unroll
+        for (int k = 1; k <= NOP; k++) {
+            const int rm = wv * RX + j - k, rp = wv * RX + j + k;
+            const int hm = j - k < 0 ? NOP + (j - k) : -1;
+            const int hp = j + k >= RX ? NOP + (j + k - RX) : -1;
+            float vm, vp;
+            if (j - k >= 0) vm = c[j - k];
+            else vm = rm >= 0 ? xr[0][rm][lane] : hg[hm];
+            if (j + k < RX) vp = c[j + k];
+            else vp = rp < ROWS ? xr[0][rp][lane] : hg[hp];
+            xm[k - 1] = vm;
+            xp[k - 1] = vp;
+        }
+        float pp = p[j] * spz;
+        pp = pp * spx[j];
+        const float v = leap2d<NOP, SSE>(a, c[j], zm, zp, xm, xp, pp, vv[j]);
+        float nv = (zint && ((xint_m >> j) & 1u)) ? v : pp;
+        for (int e = ib; e < ie; e++) {
+            const awefd_inj en = a.inj2[e];
+            const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
+            const float inj = nv + amp * en.w * en.w2 * en.w3 * vv[j];
+            nv = (zin && en.z == iz && en.x == x0 + j) ? inj : nv;
+        }
+        float w = nv * spz;
+        w = w * spx[j];
+        s1[j] = w;
+    }
+    if (nst == 1) {                               /* block-uniform */
+#pragma unroll
+        for (int j = 0; j < RX; j++) {
+            const int r = wv * RX + j;
+            if (r >= H && r < ROWS - H) bst<NT>(rD1, vzo, (unsigned)(x0 + j) * rowb, s1[j]);
+        }
+        return;
+    }
+#pragma unroll
+    for (int j = 0; j < RX; j++) xr[1][wv * RX + j][lane] = s1[j];
+    __syncthreads();
+
+    /* second step (it + 1): current field s1, previous field c */
+    float s2[RX];
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        float zm[NOP], zp[NOP], xm[NOP], xp[NOP];
+        zneigh<NOP>(s1[j], 0.f, zm, zp);          /* edge lanes are not used */
+#pragma unroll
+        for (int k = 1; k <= NOP; k++) {
+            const int rm = wv * RX + j - k, rp = wv * RX + j + k;
+            xm[k - 1] = j - k >= 0 ? s1[j - k] : xr[1][max(rm, 0)][lane];
+            xp[k - 1] = j + k < RX ? s1[j + k] : xr[1][min(rp, ROWS - 1)][lane];
+        }
+        float pp = c[j] * spz;
+        pp = pp * spx[j];
+        const float v = leap2d<NOP, SSE>(a, s1[j], zm, zp, xm, xp, pp, vv[j]);
+        float nv = (zint && ((xint_m >> j) & 1u)) ? v : pp;
+        for (int e = ib; e < ie; e++) {
+            const awefd_inj en = a.inj2[e];
+            const float amp = a.wav[(long)(it + 1) * a.wstride + (a.wexpl ? 0 : en.src)];
+            const float inj = nv + amp * en.w * en.w2 * en.w3 * vv[j];
+            nv = (zin && en.z == iz && en.x == x0 + j) ? inj : nv;
+        }
+        float w = nv * spz;
+        w = w * spx[j];
+        s2[j] = w;
+        const int r = wv * RX + j;
+        if (r >= H && r < ROWS - H) bst<NT>(rD1, vzo, (unsigned)(x0 + j) * rowb, w);
+    }
+#pragma unroll
+    for (int j = 0; j < RX; j++) xr[2][wv * RX + j][lane] = s2[j];
+    __syncthreads();
+
+    /* trace row it + 1 (the first step's field, with its pending sponge
+     * factor) for the receivers whose first corner is an output of this tile */
+    for (int k = a.rbeg3[col] + tid; k < a.rbeg3[col + 1]; k += NW * WAVE) {
+        const int r = a.rlist3[k];
+        const int jz = a.rj[3 * r], jx = a.rj[3 * r + 1];
+        float g = 0.f;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int z = jz + (q & 1), x = jx + ((q >> 1) & 1);
+            float u = xr[1][x - bx0][z - z0];
+            if (a.dabc) u = sponge<2>(u, z, x, 0, a);
+            const float t = u * a.rw[8 * r + q];
+            g = q == 0 ? t : g + t;
+        }
+        a.trow[(long)(it + 1) * a.nr + r] = g;
+    }
+
+    /* third step (it + 2): current field s2, previous field s1 */
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        float zm[NOP], zp[NOP], xm[NOP], xp[NOP];
+        zneigh<NOP>(s2[j], 0.f, zm, zp);
+#pragma unroll
+        for (int k = 1; k <= NOP; k++) {
+            const int rm = wv * RX + j - k, rp = wv * RX + j + k;
+            xm[k - 1] = j - k >= 0 ? s2[j - k] : xr[2][max(rm, 0)][lane];
+            xp[k - 1] = j + k < RX ? s2[j + k] : xr[2][min(rp, ROWS - 1)][lane];
+        }
+        float pp = s1[j] * spz;
+        pp = pp * spx[j];
+        const float v = leap2d<NOP, SSE>(a, s2[j], zm, zp, xm, xp, pp, vv[j]);
+        float nv = (zint && ((xint_m >> j) & 1u)) ? v : pp;
+        for (int e = ib; e < ie; e++) {
+            const awefd_inj en = a.inj2[e];
+            const float amp = a.wav[(long)(it + 2) * a.wstride + (a.wexpl ? 0 : en.src)];
+            const float inj = nv + amp * en.w * en.w2 * en.w3 * vv[j];
+            nv = (zin && en.z == iz && en.x == x0 + j) ? inj : nv;
+        }
+        float w = nv * spz;
+        w = w * spx[j];
+        const int r = wv * RX + j;
+        if (r >= H && r < ROWS - H) bst<NT>(rD2, vzo, (unsigned)(x0 + j) * rowb, w);
+    }
+}
+
+template <int NOP, bool SSE, int RX, int NW>
+__global__ __launch_bounds__(NW * WAVE) void k_cart_batch_t3(const awefd_cart_args *__restrict__ shots,
+                                                             const int *__restrict__ bshot, int it, int roles,
+                                                             int nst)
+{
+    const int s = __builtin_amdgcn_readfirstlane(bshot[blockIdx.x]);
+    const awefd_cart_args &a = shots[s];
+    cart_step3_body<NOP, SSE, RX, NW>(a, (int)blockIdx.x - a.blk02, it, roles, nst);
+}
+
 /* z-high one-way ABC as a separate pass, for grids whose last z tile is too
  * narrow to hold the recurrence (abcone3d_apply, fdutil.c:2110-2118).  One
  * thread per (x, y) column. */
@@ -953,6 +1167,19 @@ int launch_t2_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int 
     return (int)hipGetLastError();
 }
 
+template <int NOP, bool SSE>
+int launch_t3_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int nst, int rx,
+                  hipStream_t s)
+{
+    constexpr int NW = AWEFD_BATCH_NWAVE3;
+    switch (rx) {
+    case 4: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 4, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
+    case 8: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 8, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
 template <int NOP>
 int launch_t2_nop(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int two, int sse,
                   int rx, hipStream_t s)
@@ -1002,6 +1229,21 @@ extern "C" int awefd_launch_cart_batch_t2(const awefd_cart_args *shots, const in
     case 4: return launch_t2_nop<4>(shots, bshot, nblk, it, roles, two, sse, rx, s);
     case 5: return launch_t2_nop<5>(shots, bshot, nblk, it, roles, two, sse, rx, s);
     case 6: return launch_t2_nop<6>(shots, bshot, nblk, it, roles, two, sse, rx, s);
+    default: return (int)hipErrorInvalidValue;
+    }
+}
+
+extern "C" int awefd_launch_cart_batch_t3(const awefd_cart_args *shots, const int *bshot, int nblk, int it,
+                                          int roles, int nst, int nop, int sse, int rx, void *stream)
+{
+    hipStream_t s = (hipStream_t)stream;
+    switch (nop) {
+    case 1: return sse ? launch_t3_sse<1, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<1, false>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 2: return sse ? launch_t3_sse<2, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<2, false>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 3: return sse ? launch_t3_sse<3, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<3, false>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 4: return sse ? launch_t3_sse<4, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<4, false>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 5: return sse ? launch_t3_sse<5, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<5, false>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 6: return sse ? launch_t3_sse<6, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<6, false>(shots, bshot, nblk, it, roles, nst, rx, s);
     default: return (int)hipErrorInvalidValue;
     }
 }
