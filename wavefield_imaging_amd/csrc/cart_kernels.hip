@@ -802,7 +802,9 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
     const int col = tzi + a.ntz2 * txi;
     const int ib = a.ibeg2[col], ie = a.ibeg2[col + 1];
 
-    __shared__ float xr[3][ROWS][WAVE];
+    /* xr[0]: the current field, then (once every wave is past the first
+     * step) the second step's field; xr[1]: the first step's field */
+    __shared__ float xr[2][ROWS][WAVE];
 #pragma unroll
     for (int j = 0; j < RX; j++) xr[0][wv * RX + j][lane] = c[j];
     __syncthreads();
@@ -881,7 +883,7 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
         if (r >= H && r < ROWS - H) bst<NT>(rD1, vzo, (unsigned)(x0 + j) * rowb, w);
     }
 #pragma unroll
-    for (int j = 0; j < RX; j++) xr[2][wv * RX + j][lane] = s2[j];
+    for (int j = 0; j < RX; j++) xr[0][wv * RX + j][lane] = s2[j];
     __syncthreads();
 
     /* trace row it + 1 (the first step's field, with its pending sponge
@@ -909,8 +911,8 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
 #pragma unroll
         for (int k = 1; k <= NOP; k++) {
             const int rm = wv * RX + j - k, rp = wv * RX + j + k;
-            xm[k - 1] = j - k >= 0 ? s2[j - k] : xr[2][max(rm, 0)][lane];
-            xp[k - 1] = j + k < RX ? s2[j + k] : xr[2][min(rp, ROWS - 1)][lane];
+            xm[k - 1] = j - k >= 0 ? s2[j - k] : xr[0][max(rm, 0)][lane];
+            xp[k - 1] = j + k < RX ? s2[j + k] : xr[0][min(rp, ROWS - 1)][lane];
         }
         float pp = s1[j] * spz;
         pp = pp * spx[j];
