@@ -730,6 +730,11 @@ static int time_choice(awefd_batch *b, int depth, int rx, int group_mb, int step
 {
     int rc = batch_build(b, rx, group_mb), fp, fc;
     b->depth = depth;
+    /* whole passes plus the closing single step, as a run ends */
+    if (depth >= 2) {
+        const int k = (steps - 1) / depth > 0 ? (steps - 1) / depth : 1;
+        steps = depth * k + 1 <= b->nt ? depth * k + 1 : b->nt;
+    }
     if (!rc && depth >= 2) rc = t2_build(b);
     if (!rc) rc = batch_reset(b);
     /* warm the tables */
