@@ -968,7 +968,50 @@ __global__ __launch_bounds__(256) void k_cart_abc_zhigh(const awefd_cart_args a)
 
 /* One-way ABC on the x faces, then the y faces (abcone3d_apply,
  * fdutil.c:2122-2176), each followed by the sponge of the shell nodes it
- * finalises.  One thread per line; lines along z are coalesced. */
+ * finalises.  One thread per line; lines along z are coalesced.  A line's
+ * nop + 2 nodes at each face (the step kernel stored them unsponged) are
+ * loaded once, the serial recurrence uom[i] = um[i+-1] + (um[i] - uo[i+-1]) b
+ * and the sponge run in registers, and they are stored once.  The 3-D
+ * set-up guarantees n >= 2 nop + 4, so the two faces' node sets are
+ * disjoint. */
+template <int NOP>
+__device__ __forceinline__ void abc_line(float *__restrict__ P, const float *__restrict__ C, long b, long st, int n,
+                                         float bl, float bh, bool sp, int iz, int ix, int iy, int ax,
+                                         const awefd_cart_args &a)
+{
+    constexpr int S = NOP + 2;
+    float ul[S], uh[S], cl[S], ch[S];
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        ul[i] = P[b + i * st];
+        cl[i] = C[b + i * st];
+        uh[i] = P[b + (long)(n - S + i) * st];
+        ch[i] = C[b + (long)(n - S + i) * st];
+    }
+#pragma unroll
+    for (int k = 0; k < NOP; k++) {
+        const int l = NOP - k;                  /* low face: node NOP - k */
+        ul[l] = cl[l + 1] + (cl[l] - ul[l + 1]) * bl;
+        const int h = k + 1;                    /* high face: node n - NOP + k - 1 */
+        uh[h] = ch[h - 1] + (ch[h] - uh[h - 1]) * bh;
+    }
+    if (sp) {
+#pragma unroll
+        for (int i = 0; i < S; i++) {
+            const int jl = i, jh = n - S + i;
+            const int zl = iz, xl = ax == 1 ? jl : ix, yl = ax == 2 ? jl : iy;
+            const int xh = ax == 1 ? jh : ix, yh = ax == 2 ? jh : iy;
+            if (in_sponge<3>(zl, xl, yl, a)) ul[i] = sponge<3>(ul[i], zl, xl, yl, a);
+            if (in_sponge<3>(zl, xh, yh, a)) uh[i] = sponge<3>(uh[i], zl, xh, yh, a);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < S; i++) {
+        P[b + i * st] = ul[i];
+        P[b + (long)(n - S + i) * st] = uh[i];
+    }
+}
+
 template <int NOP>
 __global__ __launch_bounds__(256) void k_cart_abc_x(const awefd_cart_args a)
 {
@@ -976,23 +1019,10 @@ __global__ __launch_bounds__(256) void k_cart_abc_x(const awefd_cart_args a)
     if (t >= (long)a.n1 * a.n3) return;
     const int iz = (int)(t % a.n1), iy = (int)(t / a.n1);
     const long sx = a.ld, sy = (long)a.ld * a.n2;
-    const long b = iy * sy + iz;
     const float bl = a.bxl[(long)iy * a.n1 + iz], bh = a.bxh[(long)iy * a.n1 + iz];
-    float *P = a.P;
-    const float *C = a.C;
-    for (int k = 0; k < NOP; k++) {
-        long ix = NOP - k;
-        P[b + ix * sx] = C[b + (ix + 1) * sx] + (C[b + ix * sx] - P[b + (ix + 1) * sx]) * bl;
-        ix = a.n2 - NOP + k - 1;
-        P[b + ix * sx] = C[b + (ix - 1) * sx] + (C[b + ix * sx] - P[b + (ix - 1) * sx]) * bh;
-    }
-    if (!a.dabc) return;
+    /* lines in the y shell are sponged by the y pass */
     const bool ysh = iy <= NOP + 1 || iy >= a.n3 - NOP - 2;
-    if (ysh) return;
-    /* x shell: [0, NOP+2) and [n2-NOP-2, n2), each node once */
-    const int hi = max(NOP + 2, a.n2 - NOP - 2);
-    for (int ix = 0; ix < a.n2; ix = ix + 1 == NOP + 2 ? hi : ix + 1)
-        if (in_sponge<3>(iz, ix, iy, a)) P[b + ix * sx] = sponge<3>(P[b + ix * sx], iz, ix, iy, a);
+    abc_line<NOP>(a.P, a.C, iy * sy + iz, sx, a.n2, bl, bh, a.dabc && !ysh, iz, 0, iy, 1, a);
 }
 
 template <int NOP>
@@ -1002,21 +1032,8 @@ __global__ __launch_bounds__(256) void k_cart_abc_y(const awefd_cart_args a)
     if (t >= (long)a.n1 * a.n2) return;
     const int iz = (int)(t % a.n1), ix = (int)(t / a.n1);
     const long sx = a.ld, sy = (long)a.ld * a.n2;
-    const long b = ix * sx + iz;
     const float bl = a.byl[(long)ix * a.n1 + iz], bh = a.byh[(long)ix * a.n1 + iz];
-    float *P = a.P;
-    const float *C = a.C;
-    for (int k = 0; k < NOP; k++) {
-        long iy = NOP - k;
-        P[b + iy * sy] = C[b + (iy + 1) * sy] + (C[b + iy * sy] - P[b + (iy + 1) * sy]) * bl;
-        iy = a.n3 - NOP + k - 1;
-        P[b + iy * sy] = C[b + (iy - 1) * sy] + (C[b + iy * sy] - P[b + (iy - 1) * sy]) * bh;
-    }
-    if (!a.dabc) return;
-    /* y shell: [0, NOP+2) and [n3-NOP-2, n3), each node once */
-    const int hi = max(NOP + 2, a.n3 - NOP - 2);
-    for (int iy = 0; iy < a.n3; iy = iy + 1 == NOP + 2 ? hi : iy + 1)
-        if (in_sponge<3>(iz, ix, iy, a)) P[b + iy * sy] = sponge<3>(P[b + iy * sy], iz, ix, iy, a);
+    abc_line<NOP>(a.P, a.C, ix * sx + iz, sy, a.n3, bl, bh, a.dabc, iz, ix, 0, 2, a);
 }
 
 /* One face of one line of the hybrid boundary (apply_abc,
