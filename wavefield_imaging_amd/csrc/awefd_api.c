@@ -399,6 +399,10 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
         for (int a = 0; a < 3 && rc == 0; a++) {
             free(m->h_sp[a]);
             m->h_sp[a] = malloc(sizeof(float) * m->np[a]);
+            if (!m->h_sp[a]) {
+                rc = awefd__fail(AWEFD_ENOMEM, "sponge factors");
+                break;
+            }
             if (a < m->ndim) awefd_sponge_axis(w, m->nb, m->np[a], m->h_sp[a]);
             else m->h_sp[a][0] = 1.f;
             rc = upload(m, (void **)&m->dsp[a], m->h_sp[a], sizeof(float) * m->np[a]);
@@ -437,6 +441,11 @@ static int set_points(awefd_model *m, int which, int n, const float *coords)
     if (!m || n < 0 || (n > 0 && !coords)) return awefd__fail(AWEFD_EINVAL, "set_points: bad argument");
     int *j = malloc(sizeof(int) * 3 * (n > 0 ? n : 1));
     float *w = malloc(sizeof(float) * 8 * (n > 0 ? n : 1));
+    if (!j || !w) {
+        free(j);
+        free(w);
+        return awefd__fail(AWEFD_ENOMEM, "set_points: %d points", n);
+    }
     /* grid in (z, x, y) order; spherical: z = theta, x = r, y = phi */
     int npad[3];
     float opad[3], d[3];
@@ -459,13 +468,14 @@ static int set_points(awefd_model *m, int which, int n, const float *coords)
     if (m->sinc) {
         /* sinc=y: the 9-node taps replace the bilinear corners */
         sp = malloc(sizeof(*sp) * (n > 0 ? n : 1));
-        if (!sp || awefd_sinc_points(m->ndim, n, coords, npad, opad, d, sp)) {
+        const int oom = sp == NULL;
+        if (oom || awefd_sinc_points(m->ndim, n, coords, npad, opad, d, sp)) {
             free(j);
             free(w);
             free(sp);
-            return sp ? awefd__fail(AWEFD_EINVAL, "a %s's sinc taps (4 nodes each way) leave the padded "
-                                    "grid", which == 0 ? "source" : "receiver")
-                      : awefd__fail(AWEFD_ENOMEM, "sinc records");
+            return oom ? awefd__fail(AWEFD_ENOMEM, "sinc records")
+                       : awefd__fail(AWEFD_EINVAL, "a %s's sinc taps (4 nodes each way) leave the padded "
+                                     "grid", which == 0 ? "source" : "receiver");
         }
     }
     /* every interpolation corner must be a padded node: the kernels index
