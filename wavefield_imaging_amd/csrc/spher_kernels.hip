@@ -335,9 +335,9 @@ __device__ __forceinline__ void load_plane(float (&q)[RX], float (&v)[RX], float
     hal[1] = bld(rP, vz, soff(x0 + RX, p, pbase, rowb, psz));
 }
 
-/* Six waves per SIMD: 80 VGPRs (82 without the hint, five waves) for a 2 %
- * faster step on the 460 x 572 x 572 shell; seven (72 VGPRs, spills) is 20 %
- * slower. */
+/* Six waves per SIMD: 75 VGPRs and no spill (the metric factors in LDS);
+ * seven (72 VGPRs) or eight (64) spill and run 17 % and 50 % slower on the
+ * 460 x 572 x 572 shell. */
 __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args a)
 {
     constexpr int QN = 4;
@@ -378,14 +378,17 @@ __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args 
     const bool rsh = ir < W || ir >= n1 - W;
     const float m2 = zin ? a.mr2[ir] : 0.f;
     const float m1 = zin ? a.mr1[ir] : 0.f;
-    double t1[RX], t2[RX];
+    /* the double metric factors of this lane's nodes, read once per block
+     * into LDS slots only this thread uses: 16 VGPRs fewer than holding
+     * them in registers, for six waves per SIMD without spills */
+    __shared__ double mts[2][TX][WAVE];
     unsigned tint_m = 0, tsh_m = 0, tin_m = 0;
 #pragma unroll
     for (int j = 0; j < RX; j++) {
         const int ith = x0 + j;
         const bool ok = zin && ith < n2;
-        t1[j] = ok ? a.mt1[(long)ith * n1 + ir] : 0.0;
-        t2[j] = ok ? a.mt2[(long)ith * n1 + ir] : 0.0;
+        mts[0][wv * RX + j][lane] = ok ? a.mt1[(long)ith * n1 + ir] : 0.0;
+        mts[1][wv * RX + j][lane] = ok ? a.mt2[(long)ith * n1 + ir] : 0.0;
         tin_m |= ith < n2 ? 1u << j : 0u;
         tint_m |= (ith >= NOP && ith < n2 - NOP) ? 1u << j : 0u;
         tsh_m |= (ith < W || ith >= n2 - W) ? 1u << j : 0u;
@@ -435,9 +438,9 @@ __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args 
                 const float ppth = ((tp - 2.f * c) + tm) / dt2;
                 const float ppph = ((fp - 2.f * c) + fm) / dp2;
                 const float s1 = m2 * pra + ppra;
-                double acc = (double)s1 + t1[j] * (double)pth;
+                double acc = (double)s1 + mts[0][wv * RX + j][lane] * (double)pth;
                 acc = acc + (double)(m1 * ppth);
-                acc = acc + t2[j] * (double)ppph;
+                acc = acc + mts[1][wv * RX + j][lane] * (double)ppph;
                 const bool inter = rint && ((tint_m >> j) & 1u) && pint;
                 const float lap = inter ? (float)acc : 0.f;
                 float pp = bq_[B][j];
