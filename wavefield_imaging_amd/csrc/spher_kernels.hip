@@ -56,6 +56,7 @@ constexpr int TX = NWAVE * RX;
 constexpr int W = AWEFD_SPHER_SHELL;
 constexpr int NOP = AWEFD_SPHER_NOP;
 constexpr int NT = 2;                       /* non-temporal policy */
+constexpr int DTAB_LDS = 512;               /* sponge table entries staged in LDS */
 static_assert(TX == AWEFD_SPHER_TX, "tile width");
 
 /* Sponge depth of a node (spongeBC_2D/3D, spher_utils.c:641-740), -1 outside
@@ -382,6 +383,14 @@ __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args 
      * into LDS slots only this thread uses: 16 VGPRs fewer than holding
      * them in registers, for six waves per SIMD without spills */
     __shared__ double mts[2][TX][WAVE];
+    /* the sponge table (nb + 1 doubles) in LDS when it fits: a node's factor
+     * is a load whose index depends on the node, and from L2 its latency
+     * cost a quarter of the step */
+    __shared__ double dtab[DTAB_LDS];
+    const bool dlds = a.nb + 1 <= DTAB_LDS;                 /* block-uniform */
+    if (dlds)
+        for (int i = tid; i <= a.nb; i += NTHR) dtab[i] = a.damp[i];
+    __syncthreads();
     unsigned tint_m = 0, tsh_m = 0, tin_m = 0;
 #pragma unroll
     for (int j = 0; j < RX; j++) {
@@ -447,10 +456,10 @@ __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args 
                 int di = -1;
                 if (spw) {
                     di = damp_index<3>(ir, x0 + j, y, n1, n2, n3, nb);
-                    pp = damp(pp, di, a.damp);
+                    pp = dlds ? damp(pp, di, dtab) : damp(pp, di, a.damp);
                 }
                 const float f0 = (bv_[B][j] * lap + 2.f * c) - pp;
-                float f = spw ? damp(f0, di, a.damp) : f0;
+                float f = spw ? (dlds ? damp(f0, di, dtab) : damp(f0, di, a.damp)) : f0;
                 if (a.fsrf) f = 0.f;   /* freeBC_3D zeroes the whole field */
                 const bool sh = rsh || ((tsh_m >> j) & 1u) || psh;
                 const unsigned s = soff(x0 + j, y, pbase, rowb, psz);
