@@ -92,12 +92,11 @@ static int face_count(int n)
     return lo + (n - hs);
 }
 
-/* Step tiling: 64 r x TX theta columns, phi chunks sized for ~target blocks.
- * The step kernel is bound by latency rather than bandwidth: short phi chunks
- * (about 10 planes for the 460 x 572 x 572 shell at 16K blocks) keep more
- * blocks resident, which pays more than the extra halo planes they re-read
- * (measured with the metric tables in registers: 4K blocks 151, 16K 161,
- * 32K 164, 64K 162 Gcell/s; with them in LDS: 16K 172, 32K 169). */
+/* Step tiling: 64 r x TX theta columns, phi chunks sized for ~target blocks
+ * (about 20 planes for the 460 x 572 x 572 shell at 8K blocks).  Shorter
+ * chunks keep more blocks resident but re-read more halo planes; measured
+ * step times on that shell: 4K blocks 0.609 ms, 6K-12K 0.595, 16K 0.608,
+ * 32K 0.637. */
 static void spher_tiling(awefd_model *m)
 {
     const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2];
@@ -105,7 +104,7 @@ static void spher_tiling(awefd_model *m)
     m->ntx = (n2 + AWEFD_SPHER_TX - 1) / AWEFD_SPHER_TX;
     m->nty = 1;
     if (m->ndim == 3) {
-        int target = m->target_blocks > 0 ? m->target_blocks : 16384;
+        int target = m->target_blocks > 0 ? m->target_blocks : 8192;
         const char *e = getenv("AWEFD_SPHER_BLOCKS");
         if (e && atoi(e) > 0) target = atoi(e);
         const long cols = (long)m->ntz * m->ntx;
