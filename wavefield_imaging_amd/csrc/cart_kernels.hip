@@ -317,6 +317,21 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
 
     const bool zlow = NDIM == 3 && a.abc && tzi == 0 && !a.fsrf;
     const bool zhigh = NDIM == 3 && a.abc && !a.zsep && tzi == a.ntz - 1;
+    /* the z-face one-way coefficients of the block's rows and planes, staged
+     * in LDS when the chunk has at most ZB_PL planes: read from memory in the
+     * plane loop, each was a load the recurrence waited on (6 % of a cart3d
+     * step).  The loop's first barrier orders these writes before the reads. */
+    constexpr int ZB_PL = 32;
+    __shared__ float zbt[NDIM == 3 ? 2 : 1][NDIM == 3 ? ZB_PL : 1][TX];
+    const bool zbs = NDIM == 3 && (zlow || zhigh) && y1 - y0 <= ZB_PL;   /* block-uniform */
+    if (NDIM == 3 && zbs) {
+        for (int i = tid; i < (y1 - y0) * TX; i += NW * WAVE) {
+            const int yy = i / TX, xx = bx0 + i % TX;
+            const long o = (long)(y0 + yy) * n2 + min(xx, n2 - 1);
+            zbt[0][yy][i % TX] = zlow ? a.bzl[o] : 0.f;
+            zbt[1][yy][i % TX] = zhigh ? a.bzh[o] : 0.f;
+        }
+    }
 
     for (int yg = y0; yg < y1; yg += QN) {
 #pragma unroll
@@ -434,7 +449,9 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
                 for (int j = 0; j < RX; j++) {
                     if (!((xin_m >> j) & 1u)) continue;
                     if (zlow) {
-                        const float b = a.bzl[y * n2 + x0 + j];
+                        float b;
+                        if (zbs) b = zbt[0][y - y0][x0 - bx0 + j];
+                        else b = a.bzl[y * n2 + x0 + j];
 #pragma unroll
                         for (int k = 0; k < NOP; k++) {
                             const int zt = NOP - k;
@@ -444,7 +461,9 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
                         }
                     }
                     if (zhigh) {
-                        const float b = a.bzh[y * n2 + x0 + j];
+                        float b;
+                        if (zbs) b = zbt[1][y - y0][x0 - bx0 + j];
+                        else b = a.bzh[y * n2 + x0 + j];
 #pragma unroll
                         for (int k = 0; k < NOP; k++) {
                             const int zt = n1 - NOP + k - 1;
