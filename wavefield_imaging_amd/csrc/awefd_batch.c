@@ -80,7 +80,8 @@ struct awefd_batch {
     int *gnblk2;
     awefd_inj **d_inj2;              /* per shot: two-step injection buckets     */
     int **d_ibeg2;
-    int **d_rbeg3, **d_rlist3;       /* per shot: three-step receiver buckets    */
+    int **d_rbeg3;                   /* per shot: three-step receiver buckets    */
+    awefd_rx3 **d_rx3;               /* and their records                        */
     int t2_ok;                       /* multi-step tables built for the tile     */
     int t2_depth;                    /* ... and for this depth                   */
     awefd_copy_job *d_fjobs;         /* end of a depth-2 run: fields back into   */
@@ -130,14 +131,14 @@ void awefd_batch_destroy(awefd_batch *b)
         if (b->d_inj2) dfree(&b->d_inj2[s]);
         if (b->d_ibeg2) dfree(&b->d_ibeg2[s]);
         if (b->d_rbeg3) dfree(&b->d_rbeg3[s]);
-        if (b->d_rlist3) dfree(&b->d_rlist3[s]);
+        if (b->d_rx3) dfree(&b->d_rx3[s]);
     }
     free(b->e0);
     free(b->e1);
     free(b->d_inj2);
     free(b->d_ibeg2);
     free(b->d_rbeg3);
-    free(b->d_rlist3);
+    free(b->d_rx3);
     dfree(&b->d_bshot2);
     dfree(&b->d_fjobs);
     dfree(&b->d_fcbeg);
@@ -428,26 +429,34 @@ static int t2_buckets(const awefd_batch *b, const awefd_model *m, int depth, int
 
 /* Two-step tables for the current tile and groups. */
 /* receivers bucketed by the tile whose outputs hold their first corner */
-static int t3_receivers(const awefd_model *m, int sz, int sx, int ntz, int ntx, int **rbeg, int **rlist)
+static int t3_receivers(const awefd_model *m, int sz, int sx, int ntz, int ntx, int **rbeg, awefd_rx3 **rx)
 {
     const int nt = ntz * ntx, nr = m->nr;
-    int *beg = calloc(nt + 1, sizeof(int)), *list = malloc(sizeof(int) * (nr > 0 ? nr : 1));
-    if (!beg || !list) {
-        free(beg); free(list);
+    int *beg = calloc(nt + 1, sizeof(int)), *fill = malloc(sizeof(int) * (nt > 0 ? nt : 1));
+    awefd_rx3 *list = calloc(nr > 0 ? nr : 1, sizeof(*list));
+    if (!beg || !fill || !list) {
+        free(beg); free(fill); free(list);
         return awefd__fail(AWEFD_ENOMEM, "three-step receiver buckets");
     }
     for (int r = 0; r < nr; r++) beg[m->h_rj[3 * r] / sz + ntz * (m->h_rj[3 * r + 1] / sx) + 1]++;
     for (int t = 0; t < nt; t++) beg[t + 1] += beg[t];
-    int *fill = malloc(sizeof(int) * (nt > 0 ? nt : 1));
-    if (!fill) {
-        free(beg); free(list);
-        return awefd__fail(AWEFD_ENOMEM, "three-step receiver buckets");
-    }
     memcpy(fill, beg, sizeof(int) * nt);
-    for (int r = 0; r < nr; r++) list[fill[m->h_rj[3 * r] / sz + ntz * (m->h_rj[3 * r + 1] / sx)]++] = r;
+    for (int r = 0; r < nr; r++) {
+        const int jz = m->h_rj[3 * r], jx = m->h_rj[3 * r + 1];
+        awefd_rx3 *e = &list[fill[jz / sz + ntz * (jx / sx)]++];
+        e->r = r;
+        e->jz = jz;
+        e->jx = jx;
+        for (int q = 0; q < 4; q++) e->w[q] = m->h_rw[8 * r + q];
+        /* the device sponge tables hold these same floats */
+        for (int k = 0; k < 2; k++) {
+            e->sz[k] = m->sponge ? m->h_sp[0][jz + k] : 1.f;
+            e->sx[k] = m->sponge ? m->h_sp[1][jx + k] : 1.f;
+        }
+    }
     free(fill);
     *rbeg = beg;
-    *rlist = list;
+    *rx = list;
     return AWEFD_OK;
 }
 
@@ -463,10 +472,10 @@ static int t2_build(awefd_batch *b)
     if (!b->d_inj2) b->d_inj2 = calloc(ns, sizeof(awefd_inj *));
     if (!b->d_ibeg2) b->d_ibeg2 = calloc(ns, sizeof(int *));
     if (!b->d_rbeg3) b->d_rbeg3 = calloc(ns, sizeof(int *));
-    if (!b->d_rlist3) b->d_rlist3 = calloc(ns, sizeof(int *));
+    if (!b->d_rx3) b->d_rx3 = calloc(ns, sizeof(awefd_rx3 *));
     long *goff2 = calloc(ng + 1, sizeof(long));
     int *gnblk2 = calloc(ng, sizeof(int)), *nb = malloc(sizeof(int) * ns);
-    int rc = (!b->e0 || !b->e1 || !b->d_inj2 || !b->d_ibeg2 || !b->d_rbeg3 || !b->d_rlist3 || !goff2 || !gnblk2 || !nb)
+    int rc = (!b->e0 || !b->e1 || !b->d_inj2 || !b->d_ibeg2 || !b->d_rbeg3 || !b->d_rx3 || !goff2 || !gnblk2 || !nb)
                  ? awefd__fail(AWEFD_ENOMEM, "two-step tables") : AWEFD_OK;
     for (int s = 0; s < ns && !rc; s++) {
         awefd_model *m = b->m[s];
@@ -507,24 +516,25 @@ static int t2_build(awefd_batch *b)
         free(inj);
         free(ibeg);
         h->rbeg3 = NULL;
-        h->rlist3 = NULL;
+        h->rx3 = NULL;
         if (!rc && b->depth == 3) {
-            int *rbeg = NULL, *rlist = NULL;
-            rc = t3_receivers(m, sz, sx, h->ntz2, h->ntx2, &rbeg, &rlist);
+            int *rbeg = NULL;
+            awefd_rx3 *rx = NULL;
+            rc = t3_receivers(m, sz, sx, h->ntz2, h->ntx2, &rbeg, &rx);
             if (!rc) {
                 dfree(&b->d_rbeg3[s]);
-                dfree(&b->d_rlist3[s]);
+                dfree(&b->d_rx3[s]);
                 const size_t nbeg = (size_t)h->ntz2 * h->ntx2 + 1, nrl = m->nr > 0 ? m->nr : 1;
                 e = hipMalloc((void **)&b->d_rbeg3[s], sizeof(int) * nbeg);
-                if (e == hipSuccess) e = hipMalloc((void **)&b->d_rlist3[s], sizeof(int) * nrl);
+                if (e == hipSuccess) e = hipMalloc((void **)&b->d_rx3[s], sizeof(awefd_rx3) * nrl);
                 if (e == hipSuccess) e = hipMemcpy(b->d_rbeg3[s], rbeg, sizeof(int) * nbeg, hipMemcpyHostToDevice);
-                if (e == hipSuccess) e = hipMemcpy(b->d_rlist3[s], rlist, sizeof(int) * nrl, hipMemcpyHostToDevice);
+                if (e == hipSuccess) e = hipMemcpy(b->d_rx3[s], rx, sizeof(awefd_rx3) * nrl, hipMemcpyHostToDevice);
                 if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "three-step receiver upload: %s", hipGetErrorString(e));
                 h->rbeg3 = b->d_rbeg3[s];
-                h->rlist3 = b->d_rlist3[s];
+                h->rx3 = b->d_rx3[s];
             }
             free(rbeg);
-            free(rlist);
+            free(rx);
         }
     }
     for (int g = 0; g < ng && !rc; g++) goff2[g + 1] = goff2[g] + gnblk2[g];

@@ -41,6 +41,16 @@ typedef struct {
     float w[3][9];
 } awefd_sinc_pt;
 
+/* A receiver of the three-step pass's in-block extraction: its index, first
+ * corner (z, x), its four bilinear weights and the sponge factors of the
+ * corners' z and x (1 without the sponge), so that a block reads one record
+ * instead of a chain of dependent loads. */
+typedef struct {
+    int   r, jz, jx, pad;
+    float w[4];
+    float sz[2], sx[2];
+} awefd_rx3;
+
 /* Arguments of the cartesian step kernels. */
 typedef struct {
     float       *P;          /* previous field, overwritten with the new one  */
@@ -96,7 +106,8 @@ typedef struct {
     const awefd_inj *inj2;
     /* batched three-step pass: receivers per tile, by the tile owning their
      * first interpolation corner (trace row it + 1 is extracted in-block) */
-    const int *rbeg3, *rlist3;
+    const int *rbeg3;
+    const awefd_rx3 *rx3;
 } awefd_cart_args;
 
 /* Adjoint of the cartesian 2-D loop (adj_kernels.hip). */

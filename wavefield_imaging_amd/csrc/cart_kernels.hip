@@ -751,7 +751,7 @@ __global__ __launch_bounds__(NW * WAVE) void k_cart_batch_t2(const awefd_cart_ar
  * next through LDS.  The pass reads the previous and current field and V
  * once and writes the second and third steps' fields (D1, D2); the first
  * step's trace row, it + 1, is extracted in the block from LDS by the tile
- * that owns the receiver's first corner (rbeg3/rlist3), the others by the
+ * that owns the receiver's first corner (rbeg3/rx3), the others by the
  * following passes' extraction blocks.  nst = 1 takes one step (writing D1)
  * with the same tiles, for the end of the loop. */
 template <int NOP, bool SSE, int RX, int NW>
@@ -820,6 +820,8 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
     }
     const int col = tzi + a.ntz2 * txi;
     const int ib = a.ibeg2[col], ie = a.ibeg2[col + 1];
+    /* this tile's receivers for the in-block trace row, read early */
+    const int kb = a.rbeg3[col], ke = a.rbeg3[col + 1];
 
     /* xr[0]: the current field, then (once every wave is past the first
      * step) the second step's field; xr[1]: the first step's field */
@@ -907,19 +909,21 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
 
     /* trace row it + 1 (the first step's field, with its pending sponge
      * factor) for the receivers whose first corner is an output of this tile */
-    for (int k = a.rbeg3[col] + tid; k < a.rbeg3[col + 1]; k += NW * WAVE) {
-        const int r = a.rlist3[k];
-        const int jz = a.rj[3 * r], jx = a.rj[3 * r + 1];
+    for (int k = kb + tid; k < ke; k += NW * WAVE) {
+        const awefd_rx3 rc = a.rx3[k];
         float g = 0.f;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const int z = jz + (q & 1), x = jx + ((q >> 1) & 1);
+            const int z = rc.jz + (q & 1), x = rc.jx + ((q >> 1) & 1);
             float u = xr[1][x - bx0][z - z0];
-            if (a.dabc) u = sponge<2>(u, z, x, 0, a);
-            const float t = u * a.rw[8 * r + q];
+            if (a.dabc) {            /* sponge<2>: the z factor, then the x factor */
+                u = u * rc.sz[q & 1];
+                u = u * rc.sx[(q >> 1) & 1];
+            }
+            const float t = u * rc.w[q];
             g = q == 0 ? t : g + t;
         }
-        a.trow[(long)(it + 1) * a.nr + r] = g;
+        a.trow[(long)(it + 1) * a.nr + rc.r] = g;
     }
 
     /* third step (it + 2): current field s2, previous field s1 */
