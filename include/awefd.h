@@ -146,7 +146,10 @@ int awefd_get_field(awefd_model *m, float *host);
 /* Bit-exact view of the host index map (lint2d_make / lint3d_make,
  * fdutil.c:1363-1539): which = 0 sources, 1 receivers.  idx gets 3 ints per
  * point (j1, j2, j3 along the fastest..slowest axis), w gets 8 floats per
- * point in the reference corner order (4 used in 2-D). */
+ * point in the reference corner order (4 used in 2-D).
+ * which = 2: each source's injection factor in w[8*i] (the rest 0): the
+ * spherical expl=y reflectivity of inject_sources_2D/3D
+ * (spher_utils.c:153-173, 250-310), 1 otherwise; needs the model set. */
 int awefd_debug_points(const awefd_model *m, int which, int *idx, float *w);
 
 /* Per-kernel timing with HIP events on the handle's stream.  enable=1 starts

@@ -423,6 +423,11 @@ static float orc_refl3d(const float *vel, int jr, int jph, int jth, int nrpad, i
     return orc_refl(vel, (long)jph * Y + (long)jth * Z + jr, offs, 27);
 }
 
+float orc_refl_at(int ndim, const float *vel, int jr, int jth, int jph, int nrpad, int nthpad)
+{
+    return ndim == 3 ? orc_refl3d(vel, jr, jph, jth, nrpad, nthpad) : orc_refl2d(vel, jr, jth, nrpad);
+}
+
 /* ------------------------------------------------------------------ */
 /* Spherical one-way ABC rule (spher_utils.c:583-631), serial semantics. */
 #define ORC_NOP 4

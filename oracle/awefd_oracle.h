@@ -128,6 +128,11 @@ double orc_spher_damp(int i, int nb);
  * 2 th-high, 3 r-low, 4 r-high, 5 ph-low, 6 ph-high): the last rule that the
  * reference's serial sweep applies at that cell. */
 int  orc_spher3d_rule(int ir, int ith, int iph, int nrpad, int nthpad, int nphpad);
+/* Injection reflectivity of expl=y at padded node (jr, jth[, jph]) of the raw
+ * padded velocity (r fastest, then theta, then phi): |v0-vk|/(v0+vk) of the
+ * first differing neighbour in the reference's search order, 0 if none.
+ * The node's whole neighbourhood must lie inside the array. */
+float orc_refl_at(int ndim, const float *vel, int jr, int jth, int jph, int nrpad, int nthpad);
 
 /* ---- FP64 tangent-linear model, adjoint and misfit gradient of the
  * cartesian 2-D loop (oracle_adj.c).  Constant density, no free surface.

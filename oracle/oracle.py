@@ -79,6 +79,17 @@ def spher3d_rule(ir, ith, iph, nr, nth, nph):
     return lib().orc_spher3d_rule(ir, ith, iph, nr, nth, nph)
 
 
+def refl_at(vel, j):
+    """expl=y reflectivity at padded node j = (jr, jth[, jph]) of the raw
+    padded velocity vel (array axes slowest first: [phi,] theta, r)."""
+    vel = np.ascontiguousarray(vel, np.float32)
+    f = lib().orc_refl_at
+    f.restype = C.c_float
+    ndim = vel.ndim
+    jph = int(j[2]) if ndim == 3 else 0
+    return np.float32(f(ndim, _fp(vel), int(j[0]), int(j[1]), jph, vel.shape[-1], vel.shape[-2]))
+
+
 def lint2d(xz, nzpad, nxpad, ozpad, oxpad, dz, dx):
     xz = np.ascontiguousarray(xz, np.float32).reshape(-1)
     na = xz.size // 2

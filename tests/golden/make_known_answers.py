@@ -10,6 +10,17 @@ with f1 the z fraction and f2 the x fraction; the 3-D weights carry (1-f3)
 or f3 for y in the upper bit.  A point outside [o, o + (n-1)d) on any axis
 maps to node 0 with the whole weight on the second (z+1) node.
 
+Reflectivity cases (expl=y injection, spher_utils.c:153-173 in 2-D,
+250-310 in 3-D): the reference takes |v0-vk|/(v0+vk) of the FIRST
+neighbour k, in its fixed search order, whose velocity differs from the
+source node's.  Case k sets every neighbour at search position >= k to its
+own velocity and leaves the earlier ones at v0, so the answer names position
+k.  The 3-D list holds (phi+1, r-1) twice (positions 16 and 18): the second
+copy can never be the first differing neighbour, so case 18 answers with
+position 19.  The orders below are transcribed from the reference's
+else-if chains, one entry per branch, as (dr, dtheta[, dphi]) with
+x = r, z = theta, y = phi of its index expressions.
+
 Rule cases: 0 none, 1 theta-low, 2 theta-high, 3 r-low, 4 r-high, 5 phi-low,
 6 phi-high; the answer is the last rule the serial sweep applies at the node,
 worked out by hand from the else-if choice of iop (first of r, theta, phi
@@ -106,8 +117,56 @@ def rule_cases():
     return [dict(node=list(n), grid=grid, rule=r) for n, r in cases]
 
 
+# spher_utils.c:155-170, branch by branch: z+1, x+1, z+1 x+1, x-1, z-1 x-1,
+# z+1 x-1, z-1 x+1 (x = r, z = theta)
+REFL2D_ORDER = [(0, 1), (1, 0), (1, 1), (-1, 0), (-1, -1), (-1, 1), (1, -1)]
+# spher_utils.c:250-304: cardinal (252-262), diagonals (264-286), corners
+# (288-304); (dr, dtheta, dphi) with x = r, z = theta, y = phi
+REFL3D_ORDER = [
+    (0, 1, 0), (0, 0, 1), (1, 0, 0), (0, -1, 0), (0, 0, -1), (-1, 0, 0),
+    (1, 1, 0), (-1, -1, 0), (-1, 1, 0), (1, -1, 0),
+    (0, 1, 1), (0, -1, -1), (0, -1, 1), (0, 1, -1),
+    (1, 0, 1), (-1, 0, -1), (-1, 0, 1), (1, 0, -1),
+    (-1, 0, 1),
+    (1, 1, 1), (-1, 1, 1), (1, -1, 1), (-1, -1, 1),
+    (1, 1, -1), (-1, 1, -1), (1, -1, -1), (-1, -1, -1),
+]
+
+
+def _f32_refl(v0, vk):
+    """fabs((v0 - vk) / (v0 + vk)) in float32, as the reference evaluates it."""
+    import numpy as np
+    a, b = np.float32(v0), np.float32(vk)
+    return float(np.float32(abs(np.float32(np.float32(a - b) / np.float32(a + b)))))
+
+
+def refl_cases(order, grid, node):
+    """One case per search position k: neighbours at positions >= k get
+    velocity V0 + (pos + 1) * 2^20 (distinct, exact in float32), the rest V0."""
+    v0 = 1.68e8
+    cases = []
+    for k in range(len(order)):
+        early = set(tuple(o) for o in order[:k])   # these nodes keep v0
+        nbr = {}
+        for pos in range(k, len(order)):
+            off = tuple(order[pos])
+            if off not in early:
+                nbr.setdefault(off, v0 + (pos + 1) * 2.0 ** 20)
+        first = next(p for p in range(k, len(order)) if tuple(order[p]) in nbr)
+        vk = nbr[tuple(order[first])]
+        cases.append(dict(case=k, grid=grid, node=node, v0=v0,
+                          set=[list(o) + [v] for o, v in nbr.items()],
+                          position=first, r=_f32_refl(v0, vk)))
+    # homogeneous neighbourhood: nothing differs, r = 0
+    cases.append(dict(case=len(order), grid=grid, node=node, v0=v0, set=[], position=-1, r=0.0))
+    return cases
+
+
 def main():
-    data = {"lint2d": lint2d_cases(), "lint3d": lint3d_cases(), "spher3d_rule": rule_cases()}
+    data = {"lint2d": lint2d_cases(), "lint3d": lint3d_cases(), "spher3d_rule": rule_cases(),
+            # padded grids (n_r, n_theta[, n_phi]) and the source node (r, theta[, phi])
+            "refl2d": refl_cases(REFL2D_ORDER, [12, 10], [5, 4]),
+            "refl3d": refl_cases(REFL3D_ORDER, [10, 9, 8], [4, 4, 3])}
     with open(OUT, "w") as f:
         json.dump(data, f, indent=1)
     print("wrote", OUT)

@@ -796,6 +796,33 @@ def test_spher_reflectivity_injection(orc):
     np.testing.assert_allclose(tr2, r * tr_const, rtol=1e-5, atol=1e-7 * np.abs(tr_const).max())
 
 
+def _refl_grid(case):
+    """Raw padded velocity of a reflectivity known-answer case, array axes
+    slowest first ([phi,] theta, r), and the source node (r, theta[, phi])."""
+    g, node = case["grid"], case["node"]
+    vel = np.full(g[::-1], np.float32(case["v0"]), np.float32)
+    for e in case["set"]:
+        off, v = e[:-1], e[-1]
+        idx = tuple(node[a] + off[a] for a in range(len(node)))[::-1]
+        vel[idx] = np.float32(v)
+    return vel, node
+
+
+@pytest.mark.parametrize("key", ["refl2d", "refl3d"])
+def test_spher_reflectivity_search_order(orc, key):
+    """expl=y injection scale (spher_utils.c:153-173, 250-310): one case per
+    position of the reference's neighbour search order, in which that
+    position is the first whose velocity differs; the expected value names
+    the position.  Covers the 3-D order's repeated (phi+1, r-1) entry and the
+    homogeneous neighbourhood (r = 0)."""
+    cases = _known()[key]
+    assert len(cases) == (8 if key == "refl2d" else 28)
+    for case in cases:
+        vel, node = _refl_grid(case)
+        got = orc.refl_at(vel, node)
+        assert got == np.float32(case["r"]), (key, case["case"], case["position"], got, case["r"])
+
+
 def test_spher3d_rule_known_answers(orc):
     """Last serial one-way rule per node (spher_utils.c:583-631), hand-derived."""
     for case in _known()["spher3d_rule"]:

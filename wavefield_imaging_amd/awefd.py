@@ -161,7 +161,10 @@ class Model:
         return out
 
     def points(self, which):
-        n = self.ns if which == 0 else self.nr
+        """Index map of the sources (which=0) or receivers (1): (j1, j2, j3)
+        fastest first and 8 corner weights per point; which=2: each source's
+        injection factor in column 0 of the weights."""
+        n = self.nr if which == 1 else self.ns
         idx = np.zeros((max(n, 1), 3), np.int32)
         w = np.zeros((max(n, 1), 8), np.float32)
         L.check(self.lib.awefd_debug_points(self.h, which, L.iptr(idx), L.fptr(w)))

@@ -637,10 +637,24 @@ int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav)
 int awefd_debug_points(const awefd_model *m, int which, int *idx, float *w)
 {
     if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
-    const int n = which == 0 ? m->ns : m->nr;
-    const int *j = which == 0 ? m->h_sj : m->h_rj;
-    const float *ww = which == 0 ? m->h_sw : m->h_rw;
+    if (which < 0 || which > 2) return awefd__fail(AWEFD_EINVAL, "awefd_debug_points: which=%d", which);
+    const int n = which != 1 ? m->ns : m->nr;
+    const int *j = which != 1 ? m->h_sj : m->h_rj;
+    const float *ww = which != 1 ? m->h_sw : m->h_rw;
     if (!j) return awefd__fail(AWEFD_ESTATE, "points not set");
+    if (which == 2) {
+        /* per-source injection factor: spherical expl=y reflectivity, else 1 */
+        for (int a = 0; a < n; a++) {
+            float f = 1.f;
+            if (!is_cart(m)) {
+                f = awefd__spher_src_factor(m, a);
+                if (f < 0.f) return awefd__fail(AWEFD_ESTATE, "awefd_debug_points: model not set");
+            }
+            for (int q = 0; q < 8; q++) w[8 * a + q] = q == 0 ? f : 0.f;
+            idx[3 * a] = idx[3 * a + 1] = idx[3 * a + 2] = 0;
+        }
+        return AWEFD_OK;
+    }
     /* report (fastest, middle, slowest) axis indices of the model */
     for (int a = 0; a < n; a++) {
         if (is_cart(m)) {

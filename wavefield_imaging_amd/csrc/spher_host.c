@@ -328,6 +328,15 @@ static float reflectivity(const awefd_model *m, const float *vp, int jr, int jt,
     return 0.f;
 }
 
+float awefd__spher_src_factor(const awefd_model *m, int src)
+{
+    const spher_state *s = (const spher_state *)m->spher;
+    if (!m->p.expl) return 1.f;
+    if (!s || !s->h_velp) return -1.f;
+    const int jt = m->h_sj[3 * src], jr = m->h_sj[3 * src + 1], jp = m->ndim == 3 ? m->h_sj[3 * src + 2] : 0;
+    return reflectivity(m, s->h_velp, jr, jt, jp);
+}
+
 typedef struct {
     long node;
     int order;
