@@ -10,6 +10,12 @@ hand-written adjoint kernels from those checkpoints (awefd_adjoint_grad in
 the C ABI), so ``loss.backward()`` gives dloss/dslow for any loss of the
 traces.  The reference has no gradient; this layer has no counterpart there.
 
+The checkpoints live in the model, one set at a time.  Each forward call is
+stamped; when a backward belongs to an earlier forward than the one whose
+checkpoints are live (two calls of one module before one backward, e.g.
+f(m(s1)) + f(m(s2))), it first runs its own forward again from the saved
+slowness, so every branch gets the gradient at its own model.
+
 The binding is ctypes over libawefd; torch provides device memory and the
 autograd graph only.  The library works on its own HIP stream and returns
 after synchronising it; torch's current stream is synchronised before each
@@ -27,27 +33,43 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr())
 
 
+def _forward_ckpt(owner, slow):
+    """Set the model from squared slowness and run the checkpointed forward
+    loop; returns the (nt, nr) traces."""
+    m = owner.model
+    # the library takes velocity; v = s^(-1/2), formed in float64
+    vel = slow.detach().double().rsqrt().float().cpu().numpy()
+    m.set_model(vel)
+    tr = torch.empty((m.nt, m.nr), dtype=torch.float32, device=owner.device)
+    torch.cuda.current_stream(owner.device).synchronize()
+    L.check(m.lib.awefd_run_ckpt(m.h, _ptr(tr), int(owner.ckpt)))
+    return tr
+
+
 class _TracesFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, slow, owner):
-        m = owner.model
         if slow.dtype != torch.float32 or slow.device != owner.device:
             raise ValueError("slowness must be float32 on %s" % owner.device)
         if tuple(slow.shape) != owner.shape:
             raise ValueError("slowness shape %s, the model is %s" % (tuple(slow.shape), owner.shape))
-        # the library takes velocity; v = s^(-1/2), formed in float64
-        vel = slow.detach().double().rsqrt().float().cpu().numpy()
-        m.set_model(vel)
-        tr = torch.empty((m.nt, m.nr), dtype=torch.float32, device=owner.device)
-        torch.cuda.current_stream(owner.device).synchronize()
-        L.check(m.lib.awefd_run_ckpt(m.h, _ptr(tr), int(owner.ckpt)))
+        tr = _forward_ckpt(owner, slow)
+        owner._calls += 1
+        owner._live = owner._calls
+        ctx.stamp = owner._live
         ctx.owner = owner
+        ctx.save_for_backward(slow)
         return tr
 
     @staticmethod
     def backward(ctx, gtr):
         owner = ctx.owner
         m = owner.model
+        if owner._live != ctx.stamp:
+            # a later forward replaced the checkpoints: rebuild this call's
+            (slow,) = ctx.saved_tensors
+            _forward_ckpt(owner, slow)
+            owner._live = ctx.stamp
         r = gtr.detach().to(torch.float32).contiguous()
         g = torch.empty(owner.shape, dtype=torch.float32, device=owner.device)
         torch.cuda.current_stream(owner.device).synchronize()
@@ -70,6 +92,8 @@ class Traces(torch.nn.Module):
         self.model.set_wavelet(wav)
         self.ckpt = int(ckpt)
         self.shape = (int(par["n"][1]), int(par["n"][0]))
+        self._calls = 0          # forward calls so far
+        self._live = 0           # the call whose checkpoints the model holds
         dev = self.model.lib.awefd_device(self.model.h)
         self.device = torch.device("cuda", dev)
 
@@ -77,8 +101,9 @@ class Traces(torch.nn.Module):
         return _TracesFn.apply(slow, self)
 
     def gradient(self, residual):
-        """dJ/ds for trace weights ``residual`` (nt, nr) at the model of the
-        last forward call, without an autograd graph."""
+        """dJ/ds for trace weights ``residual`` (nt, nr) at the model whose
+        checkpoints the model holds (the last forward call, or the call the
+        last backward rebuilt), without an autograd graph."""
         m = self.model
         r = residual.detach().to(device=self.device, dtype=torch.float32).contiguous()
         g = torch.empty(self.shape, dtype=torch.float32, device=self.device)

@@ -92,6 +92,9 @@ struct awefd_batch {
     hipGraphExec_t xfwd, xbwd;
     int fwd_ok, bwd_ok;
     float *bwd_gout;
+    unsigned *mgen;                  /* each model's generation when the tables
+                                        were built (setters bump it)           */
+    int built;                       /* tables built for the current tile       */
 };
 
 static void dfree(void *pp)
@@ -154,6 +157,7 @@ void awefd_batch_destroy(awefd_batch *b)
     dfree(&b->d_bshot);
     if (b->stream) hipStreamDestroy(b->stream);
     free(b->m);
+    free(b->mgen);
     free(b->u0);
     free(b->u1);
     free(b->goff);
@@ -161,36 +165,42 @@ void awefd_batch_destroy(awefd_batch *b)
     free(b);
 }
 
+/* A shot the batched kernels can run: set up, plain 2-D, like shot 0. */
+static int shot_ok(const awefd_model *m, const awefd_model *m0, int s)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "batch: shot %d is NULL", s);
+    if (m->p.kind != AWEFD_CART2D)
+        return awefd__fail(AWEFD_EINVAL, "batch: shot %d is not a cartesian 2-D model", s);
+    if (m->p.nt != m0->p.nt || m->nop != m0->nop || m->p.sse != m0->p.sse || m->dev != m0->dev)
+        return awefd__fail(AWEFD_EINVAL, "batch: shot %d differs from shot 0 in nt, fdorder, sse or device", s);
+    if (m->drho && !m->p.cden)
+        return awefd__fail(AWEFD_EINVAL, "batch: shot %d has variable density (not batched)", s);
+    if (m->p.adj || m->hyb || m->sinc)
+        return awefd__fail(AWEFD_EINVAL, "batch: shot %d uses adj=y, hybridbc=y or sinc=y (not batched)", s);
+    if (!m->model_set || !m->src_set || !m->rec_set || !m->wav_set)
+        return awefd__fail(AWEFD_ESTATE, "batch: shot %d is not fully set up", s);
+    const long need = (long)m->p.nt * (m->p.expl ? 1 : m->ns);
+    if (m->nwav < need) return awefd__fail(AWEFD_EINVAL, "batch: shot %d wavelet too short", s);
+    return AWEFD_OK;
+}
+
 int awefd_batch_create(int nshot, awefd_model *const *models, awefd_batch **out)
 {
     if (!out || nshot < 1 || !models) return awefd__fail(AWEFD_EINVAL, "awefd_batch_create: bad argument");
     *out = NULL;
-    const awefd_model *m0 = models[0];
-    for (int s = 0; s < nshot; s++) {
-        const awefd_model *m = models[s];
-        if (!m) return awefd__fail(AWEFD_EINVAL, "batch: shot %d is NULL", s);
-        if (m->p.kind != AWEFD_CART2D)
-            return awefd__fail(AWEFD_EINVAL, "batch: shot %d is not a cartesian 2-D model", s);
-        if (m->p.nt != m0->p.nt || m->nop != m0->nop || m->p.sse != m0->p.sse || m->dev != m0->dev)
-            return awefd__fail(AWEFD_EINVAL, "batch: shot %d differs from shot 0 in nt, fdorder, sse or device", s);
-        if (m->drho && !m->p.cden)
-            return awefd__fail(AWEFD_EINVAL, "batch: shot %d has variable density (not batched)", s);
-        if (m->p.adj || m->hyb || m->sinc)
-            return awefd__fail(AWEFD_EINVAL, "batch: shot %d uses adj=y, hybridbc=y or sinc=y (not batched)", s);
-        if (!m->model_set || !m->src_set || !m->rec_set || !m->wav_set)
-            return awefd__fail(AWEFD_ESTATE, "batch: shot %d is not fully set up", s);
-        const long need = (long)m->p.nt * (m->p.expl ? 1 : m->ns);
-        if (m->nwav < need) return awefd__fail(AWEFD_EINVAL, "batch: shot %d wavelet too short", s);
-    }
+    int rc0;
+    for (int s = 0; s < nshot; s++)
+        if ((rc0 = shot_ok(models[s], models[0], s))) return rc0;
     awefd_batch *b = calloc(1, sizeof *b);
     if (!b) return awefd__fail(AWEFD_ENOMEM, "batch");
     b->nshot = nshot;
     b->m = malloc(sizeof(*b->m) * nshot);
+    b->mgen = calloc(nshot, sizeof(*b->mgen));
     b->d_inj = calloc(nshot, sizeof(*b->d_inj));
     b->d_ibeg = calloc(nshot, sizeof(*b->d_ibeg));
     b->u0 = malloc(sizeof(*b->u0) * nshot);
     b->u1 = malloc(sizeof(*b->u1) * nshot);
-    if (!b->m || !b->d_inj || !b->d_ibeg || !b->u0 || !b->u1) {
+    if (!b->m || !b->mgen || !b->d_inj || !b->d_ibeg || !b->u0 || !b->u1) {
         awefd_batch_destroy(b);
         return awefd__fail(AWEFD_ENOMEM, "batch");
     }
@@ -199,6 +209,7 @@ int awefd_batch_create(int nshot, awefd_model *const *models, awefd_batch **out)
         b->u0[s] = models[s]->dP;
         b->u1[s] = models[s]->dC;
     }
+    const awefd_model *m0 = models[0];
     b->nt = m0->p.nt;
     b->nop = m0->nop;
     b->sse = m0->p.sse;
@@ -311,7 +322,10 @@ static int batch_build(awefd_batch *b, int rx, int group_mb)
         b->ngroup = ng;
         b->rx = rx;
         b->group_mb = group_mb;
+        for (int s = 0; s < b->nshot; s++) b->mgen[s] = b->m[s]->gen;
+        b->built = 1;
     } else {
+        b->built = 0;
         free(goff);
         free(gnblk);
         free(h);
@@ -322,6 +336,33 @@ static int batch_build(awefd_batch *b, int rx, int group_mb)
     b->t2_ok = 0;
     if (!rc && b->depth >= 2 && rx != 4 && rx != 8) b->depth = 1;
     return rc;
+}
+
+/* The tables and the captured graphs hold each model's device pointers
+ * (sponge tables, (v dt)^2, wavelet, points, trace buffer).  A setter that
+ * ran on a model since they were built may have replaced those buffers (its
+ * generation moved): check the shots again and rebuild everything for the
+ * current tile before the next launch. */
+static int shots_ready(awefd_batch *b)
+{
+    int rc;
+    for (int s = 0; s < b->nshot; s++) {
+        if ((rc = shot_ok(b->m[s], b->m[0], s))) return rc;
+        if ((rc = awefd__ensure_traces(b->m[s]))) return rc;
+    }
+    return AWEFD_OK;
+}
+
+static int batch_sync(awefd_batch *b)
+{
+    int stale = !b->built, rc;
+    if ((rc = shots_ready(b))) return rc;
+    for (int s = 0; s < b->nshot; s++) stale |= b->mgen[s] != b->m[s]->gen;
+    if (!stale) return AWEFD_OK;
+    const int depth = b->depth;
+    if ((rc = batch_build(b, b->rx ? b->rx : 8, b->group_mb))) return rc;
+    if (depth >= 2 && (b->rx == 4 || b->rx == 8)) b->depth = depth;
+    return AWEFD_OK;
 }
 
 static int batch_reset(awefd_batch *b)
@@ -673,7 +714,7 @@ int awefd_batch_set_depth(awefd_batch *b, int depth)
     if (depth < 1 || depth > 3) return awefd__fail(AWEFD_EINVAL, "batch depth %d (1, 2 or 3)", depth);
     HIPCHK(hipSetDevice(b->dev));
     int rc;
-    if (!b->rx && (rc = batch_build(b, 8, 0))) return rc;
+    if ((rc = batch_sync(b))) return rc;
     int sz, sx, rows, halo;
     if (depth >= 2 && !tn_tile(b, depth, &sz, &sx, &rows, &halo))
         return awefd__fail(AWEFD_EINVAL, "%d steps per pass need tile rx 4 or 8 (tile is %d)", depth, b->rx);
@@ -690,7 +731,8 @@ int awefd_batch_set_tile(awefd_batch *b, int rx, int group_mb)
     if (!ok) return awefd__fail(AWEFD_EINVAL, "batch tile: rx=%d (2, 4, 8 or 16)", rx);
     if (group_mb < 0) return awefd__fail(AWEFD_EINVAL, "batch group budget %d MiB", group_mb);
     HIPCHK(hipSetDevice(b->dev));
-    return batch_build(b, rx, group_mb);
+    int rc = shots_ready(b);
+    return rc ? rc : batch_build(b, rx, group_mb);
 }
 
 int awefd_batch_tune(awefd_batch *b, int steps, int *rx_out, int *group_out, double *ms_out)
@@ -699,10 +741,12 @@ int awefd_batch_tune(awefd_batch *b, int steps, int *rx_out, int *group_out, dou
     HIPCHK(hipSetDevice(b->dev));
     if (steps < 1) steps = 1;
     if (steps > b->nt) steps = b->nt;
+    int rc = shots_ready(b);
+    if (rc) return rc;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    int bi = -1, bg = -1, rc = AWEFD_OK;
+    int bi = -1, bg = -1;
     double bestms = 0;
     for (int i = 0; i < 4 && !rc; i++)
         for (int g = 0; g < NGC && !rc; g++) {
@@ -768,10 +812,12 @@ int awefd_batch_tune_depth(awefd_batch *b, int steps, int *rx_out, int *group_ou
     HIPCHK(hipSetDevice(b->dev));
     if (steps < 2) steps = 2;
     if (steps > b->nt) steps = b->nt;
+    int rc = shots_ready(b);
+    if (rc) return rc;
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    int bd = 0, bi = -1, bg = -1, rc = AWEFD_OK;
+    int bd = 0, bi = -1, bg = -1;
     double best = 0;
     memset(b->tune_ms, 0, sizeof b->tune_ms);
     memset(b->tune2_ms, 0, sizeof b->tune2_ms);
@@ -857,7 +903,7 @@ int awefd_batch_run(awefd_batch *b)
     if (!b) return awefd__fail(AWEFD_EINVAL, "NULL batch");
     HIPCHK(hipSetDevice(b->dev));
     int rc;
-    if (!b->rx && (rc = batch_build(b, 8, 0))) return rc;
+    if ((rc = batch_sync(b))) return rc;
     if ((rc = batch_capture(b))) return rc;
     HIPCHK(hipGraphLaunch(b->gexec, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -984,7 +1030,7 @@ int awefd_batch_run_ckpt(awefd_batch *b, int ckpt)
     for (int s = 0; s < b->nshot; s++)
         if ((rc = awefd__adj_ckpt_setup(b->m[s], ckpt))) return rc;
     const int K = awefd__adj_K(b->m[0]);
-    if (!b->rx && (rc = batch_build(b, 8, 0))) return rc;
+    if ((rc = batch_sync(b))) return rc;
     if (K != b->K) {
         b->K = K;
         b->nck = (b->nt + K - 1) / K;
