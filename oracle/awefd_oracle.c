@@ -532,6 +532,66 @@ int orc_cart2d_setup_f64(const orc_par *p, const float *vel_in, const double *sl
 
 /* ------------------------------------------------------------------ */
 /* Propagators, instantiated for float and double. */
+#ifdef ORC_REFFORM
+/* Reference-form timing build (liborc_ref.so, -O3): the constant-density
+ * step in ts_kernel.c:78-138 (2-D) / 205-227 (3-D) form, 4-wide SSE
+ * intrinsics over z with schedule(static,1) over x columns (2-D) or y planes
+ * (3-D), as the reference compiles it.  Every lane does the scalar SSE-order
+ * operations of orc_step_cart (lap from c0*u, then per k the z, x[, y] pair
+ * terms; u0 = (2u - u0) + vel*lap), so the bits are the checker's; the z
+ * remainder of a column (when (nz - 2 nop) % 4 != 0) runs scalar instead of
+ * overrunning into the halo. */
+#include <xmmintrin.h>
+static int orc_step_sse(float *u0, const float *u1, const float *vel, const float *rho, const float *c2,
+                        int nop, int ndim, int nz, int nx, int ny, int sse)
+{
+    if (!sse || rho) return 0;
+    const long sx = nz, sy = (long)nz * nx;
+    const float *cz = c2, *cx = c2 + nop, *cy = c2 + 2 * nop;
+    const float c0 = c2[0];
+    const int nouter = ndim == 3 ? ny - 2 * nop : nx - 2 * nop;
+#pragma omp parallel for schedule(static, 1)
+    for (int o = 0; o < nouter; o++) {
+        const int y0 = ndim == 3 ? nop + o : 0, y1 = ndim == 3 ? y0 + 1 : 1;
+        const int x0 = ndim == 3 ? nop : nop + o, x1 = ndim == 3 ? nx - nop : x0 + 1;
+        for (int iy = y0; iy < y1; iy++)
+            for (int ix = x0; ix < x1; ix++) {
+                const long b = iy * sy + ix * sx;
+                int iz = nop;
+                for (; iz + 4 <= nz - nop; iz += 4) {
+                    const long i = b + iz;
+                    __m128 uc = _mm_loadu_ps(u1 + i);
+                    __m128 un = _mm_sub_ps(_mm_mul_ps(_mm_set1_ps(2.f), uc), _mm_loadu_ps(u0 + i));
+                    __m128 lap = _mm_mul_ps(uc, _mm_set1_ps(c0));
+                    for (int k = 1; k <= nop; k++) {
+                        lap = _mm_add_ps(lap, _mm_mul_ps(_mm_set1_ps(cz[k]),
+                                                         _mm_add_ps(_mm_loadu_ps(u1 + i + k), _mm_loadu_ps(u1 + i - k))));
+                        lap = _mm_add_ps(lap, _mm_mul_ps(_mm_set1_ps(cx[k]), _mm_add_ps(_mm_loadu_ps(u1 + i + k * sx),
+                                                                                        _mm_loadu_ps(u1 + i - k * sx))));
+                        if (ndim == 3)
+                            lap = _mm_add_ps(lap, _mm_mul_ps(_mm_set1_ps(cy[k]), _mm_add_ps(_mm_loadu_ps(u1 + i + k * sy),
+                                                                                            _mm_loadu_ps(u1 + i - k * sy))));
+                    }
+                    _mm_storeu_ps(u0 + i, _mm_add_ps(un, _mm_mul_ps(_mm_loadu_ps(vel + i), lap)));
+                }
+                for (; iz < nz - nop; iz++) {
+                    const long i = b + iz;
+                    float lap = u1[i] * c0;
+                    for (int k = 1; k <= nop; k++) {
+                        lap = lap + cz[k] * (u1[i + k] + u1[i - k]);
+                        lap = lap + cx[k] * (u1[i + k * sx] + u1[i - k * sx]);
+                        if (ndim == 3) lap = lap + cy[k] * (u1[i + k * sy] + u1[i - k * sy]);
+                    }
+                    const float t = 2.f * u1[i] - u0[i];
+                    u0[i] = t + vel[i] * lap;
+                }
+            }
+    }
+    return 1;
+}
+#define ORC_STEP_FAST orc_step_sse
+#endif
+
 #define REAL float
 #define SFX(name) name##_f32
 #define ORC_FDCOEF(nop, ndim, d, optfd, order, out) orc_fdcoef(nop, ndim, d, optfd, order, out)
@@ -542,6 +602,7 @@ int orc_cart2d_setup_f64(const orc_par *p, const float *vel_in, const double *sl
 #undef ORC_FDCOEF
 #undef ORC_SNAP
 
+#undef ORC_STEP_FAST
 #define REAL double
 #define SFX(name) name##_f64
 #define ORC_FDCOEF(nop, ndim, d, optfd, order, out) orc_fdcoef_d(nop, ndim, d, optfd, order, out)

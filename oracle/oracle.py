@@ -13,6 +13,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_LIB_REF = None
 
 KINDS = ("cart2d", "cart3d", "spher2d", "spher3d")
 
@@ -32,8 +33,18 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _LIB
+def lib(ref_form=False):
+    """The checker build (liborc.so), or with ref_form=True the
+    reference-form timing build of the same code (liborc_ref.so: -O3, the
+    4-wide SSE step with schedule(static,1)), used for CPU baselines."""
+    global _LIB, _LIB_REF
+    if ref_form:
+        if _LIB_REF is None:
+            path = os.path.join(_HERE, "liborc_ref.so")
+            if not os.path.exists(path):
+                build()
+            _LIB_REF = C.CDLL(path)
+        return _LIB_REF
     if _LIB is None:
         path = os.path.join(_HERE, "liborc.so")
         if not os.path.exists(path):
@@ -128,7 +139,7 @@ def padded_shape(kind, n, nb=None, fdorder=4, hybrid=False):
     return tuple(n[i] + 2 * b for i in range(dims)), b
 
 
-def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, snaps=False):
+def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, snaps=False, ref_form=False):
     """Run one oracle propagation.
 
     par keys: n (list), o, d, nt, dt, nb, fdorder, optfd, dabc, fsrf, expl, cden,
@@ -187,7 +198,7 @@ def run(kind, par, vel, sou, rec, wav, den=None, precision="f32", final=True, sn
         shots.append((it, np.ctypeslib.as_array(u, shape=(ncell,)).copy().reshape(shp[::-1])))
 
     cb = CB(_cb) if snaps else CB()
-    fn = getattr(lib(), "orc_%s_%s" % (kind, precision))
+    fn = getattr(lib(ref_form), "orc_%s_%s" % (kind, precision))
     if kind.startswith("cart"):
         dn = None if den is None else np.ascontiguousarray(den, np.float32)
         rc = fn(C.byref(p), _fp(vel), _fp(dn) if dn is not None else None, _fp(sou), _fp(rec),

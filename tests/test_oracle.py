@@ -856,3 +856,31 @@ def test_fp64_error_small(orc, kind):
     b, fb, _ = orc.run(kind, par, vel, sou, rec, wav, precision="f64")
     assert np.abs(b).max() > 0
     assert _rel(a, b) < 1e-4 and _rel(fa, fb) < 1e-4, (_rel(a, b), _rel(fa, fb))
+
+
+@pytest.mark.parametrize("kind,n,nb,kw", [
+    ("cart2d", [37, 52], 6, {}),                       # (nz - 2 nop) % 4 != 0: scalar tail
+    ("cart2d", [40, 44], 8, dict(fdorder=8, free=True)),
+    ("cart3d", [21, 18, 16], 4, {}),
+    ("cart3d", [20, 17, 15], 6, dict(fdorder=6, optfd=True)),
+    ("cart3d", [19, 16, 16], 4, dict(sse=False)),     # the scalar order keeps the checker's step
+])
+def test_reference_form_build_is_bit_identical(orc, kind, n, nb, kw):
+    """liborc_ref.so (-O3, the 4-wide SSE step of ts_kernel.c with
+    schedule(static,1), used only to time the CPU baseline) gives the
+    checker's traces and final field bit for bit."""
+    ndim = len(n)
+    rng = np.random.default_rng(sum(n))
+    vel = rng.uniform(1.2e8, 3.0e8, n[::-1]).astype(np.float32)
+    h = 5.0
+    ext = [(k - 1) * h for k in n]
+    sou = np.float32([[ext[1] * 0.5, ext[0] * 0.4] if ndim == 2 else [ext[1] * 0.5, ext[2] * 0.4, ext[0] * 0.3]])
+    rec = np.float32([[ext[1] * f, ext[0] * 0.2] if ndim == 2 else [ext[1] * f, ext[2] * 0.6, ext[0] * 0.2]
+                      for f in np.linspace(0.05, 0.95, 7)])
+    nt = 60
+    par = dict(n=n, o=[0.0] * ndim, d=[h] * ndim, nt=nt, dt=synth.DT, nb=nb, dabc=True, **kw)
+    wav = synth.ricker(nt, synth.DT, freq=8e6)
+    a = orc.run(kind, par, vel, sou, rec, wav)
+    b = orc.run(kind, par, vel, sou, rec, wav, ref_form=True)
+    assert np.any(a[0])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
