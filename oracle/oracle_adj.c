@@ -43,6 +43,7 @@ static void lap2d(const orc_cart2d_f64_t *g, const double *u, double *out)
 {
     const int nz = g->nz, nx = g->nx, nop = g->nop;
     memset(out, 0, sizeof(double) * (size_t)nz * nx);
+#pragma omp parallel for schedule(static)
     for (int ix = nop; ix < nx - nop; ix++)
         for (int iz = nop; iz < nz - nop; iz++) {
             const long i = (long)ix * nz + iz;
@@ -54,23 +55,24 @@ static void lap2d(const orc_cart2d_f64_t *g, const double *u, double *out)
 }
 
 /* transpose of "keep interior rows of L": out = L^T y for y supported on the
- * interior (L is symmetric node to node) */
+ * interior.  L is symmetric node to node and y vanishes off the interior, so
+ * out_i gathers c0 y_i and c_k times the y of each neighbour that exists in
+ * the array (a z neighbour in the same column); each node is written once. */
 static void lapT2d(const orc_cart2d_f64_t *g, const double *y, double *out)
 {
     const int nz = g->nz, nx = g->nx, nop = g->nop;
-    memset(out, 0, sizeof(double) * (size_t)nz * nx);
-    for (int ix = nop; ix < nx - nop; ix++)
-        for (int iz = nop; iz < nz - nop; iz++) {
+#pragma omp parallel for schedule(static)
+    for (int ix = 0; ix < nx; ix++)
+        for (int iz = 0; iz < nz; iz++) {
             const long i = (long)ix * nz + iz;
-            const double v = y[i];
-            if (v == 0.0) continue;
-            out[i] += g->c2[0] * v;
+            double o = g->c2[0] * y[i];
             for (int k = 1; k <= nop; k++) {
-                out[i + k] += g->c2[k] * v;
-                out[i - k] += g->c2[k] * v;
-                out[i + (long)k * nz] += g->c2[nop + k] * v;
-                out[i - (long)k * nz] += g->c2[nop + k] * v;
+                if (iz - k >= 0) o += g->c2[k] * y[i - k];
+                if (iz + k < nz) o += g->c2[k] * y[i + k];
+                if (ix - k >= 0) o += g->c2[nop + k] * y[i - (long)k * nz];
+                if (ix + k < nx) o += g->c2[nop + k] * y[i + (long)k * nz];
             }
+            out[i] = o;
         }
 }
 
@@ -108,6 +110,7 @@ static void forward(const orc_cart2d_f64_t *g, const float *wav, double *dat, do
         }
         lap2d(g, c, lc);
         if (dV) lap2d(g, dc, ldc);
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < N; i++) {
             double a = interior(g, i) ? 2 * c[i] - pp[i] + g->V[i] * lc[i] : pp[i];
             cn[i] = a;
@@ -120,6 +123,7 @@ static void forward(const orc_cart2d_f64_t *g, const float *wav, double *dat, do
                 cn[node] += w * g->V[node];
                 if (dV) dcn[node] += w * dV[node];
             }
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < N; i++) {
             pp[i] = g->G[i] * c[i];
             c[i] = g->G[i] * cn[i];
@@ -175,12 +179,14 @@ int orc_cart2d_grad_f64(const orc_par *p, const float *vel, const double *slow, 
     if (dat_out) memcpy(dat_out, dat, sizeof(double) * (size_t)g.nt * g.nr);
     for (int n = g.nt - 1; n >= 0; n--) {
         /* t = G alpha_{n+1} (alpha_nt = 0) */
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < N; i++) {
             t[i] = g.G[i] * al[i];
             y[i] = interior(&g, i) ? g.V[i] * t[i] : 0.0;
         }
         lapT2d(&g, y, ly);
         lap2d(&g, cs + (size_t)n * N, lc);
+#pragma omp parallel for schedule(static)
         for (long i = 0; i < N; i++) {
             const int in = interior(&g, i);
             if (in) gV[i] += lc[i] * t[i];

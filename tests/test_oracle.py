@@ -884,3 +884,25 @@ def test_reference_form_build_is_bit_identical(orc, kind, n, nb, kw):
     b = orc.run(kind, par, vel, sou, rec, wav, ref_form=True)
     assert np.any(a[0])
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("ndim,expl", [(2, True), (3, False), (3, True)])
+def test_reference_form_spherical_nest_is_bit_identical(orc, ndim, expl):
+    """The checker's spherical solve visits nodes r fastest with the trig per
+    theta; liborc_ref.so keeps the reference's r-outer nest with the trig per
+    node (spher_utils.c:390-398).  Same values, same bits."""
+    n, nb = [22, 18, 16][:ndim], 5
+    o, d = _spher_grid(ndim, n, nb)
+    rng = np.random.default_rng(3 + ndim)
+    vel = rng.uniform(1.2e8, 3.0e8, n[::-1]).astype(np.float32)
+    c = [11.3, 9.6, 7.2][:ndim]
+    sou = np.float32([[o[0] + c[0] * d[0], o[1] + c[1] * d[1]] if ndim == 2 else
+                      [o[0] + c[0] * d[0], o[2] + c[2] * d[2], o[1] + c[1] * d[1]]])
+    nt = 40
+    par = dict(n=n, o=o, d=d, nt=nt, dt=synth.DT, nb=nb, expl=expl)
+    wav = synth.ricker(nt, synth.DT, freq=8e6)
+    kind = "spher%dd" % ndim
+    a = orc.run(kind, par, vel, sou, sou, wav)
+    b = orc.run(kind, par, vel, sou, sou, wav, ref_form=True)
+    assert np.any(a[1])
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
