@@ -230,6 +230,28 @@ int awefd_adjoint_grad(awefd_model *m, const float *d_res, float *d_grad);
 /* Free the checkpoint and adjoint buffers of a model. */
 void awefd_adjoint_release(awefd_model *m);
 
+/* ---- Allocation-free run.  No counterpart in the reference, whose
+ * programs allocate per run: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). ---- */
+
+/* Largest trace block (nt * nr * 4 bytes) awefd_run_noalloc can stage. */
+#define AWEFD_NOALLOC_TRACE_BYTES (32u << 20)
+
+/* awefd_run without snapshots (MAWEFDcart2Dcpu.c:297-383, MAWEFDcart3Dcpu.c:295-366,
+ * MAWEFDspher2Dcpu.c:264-317, MAWEFDspher3Dcpu.c:302-359; same traces, same
+ * final field), allocating nothing: the receiver traces are staged in the
+ * library's fixed device scratch (one per GPU, AWEFD_NOALLOC_TRACE_BYTES,
+ * held until the process ends) instead of the handle's trace buffer, the
+ * steps are launched one by one (no graph), and every table the loop reads
+ * was built by the setters.  traces: host nt*nr floats (may be NULL); they
+ * are returned only here (awefd_get_traces does not see them).
+ * AWEFD_EINVAL if nt*nr*4 exceeds the scratch or profiling is on.
+ * Every call on one GPU uses that GPU's one scratch: calls that overlap on
+ * one GPU (threads, separate handles) race and must be serialised by the
+ * caller. */
+int awefd_run_noalloc(awefd_model *m, float *traces);
+
 #ifdef __cplusplus
 }
 #endif

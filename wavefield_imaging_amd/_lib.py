@@ -24,7 +24,7 @@ EXPORTS = (
     "awefd_destroy", "awefd_get_traces", "awefd_batch_create", "awefd_batch_set_tile",
     "awefd_batch_tune", "awefd_batch_run", "awefd_batch_destroy", "awefd_run_ckpt",
     "awefd_adjoint_grad", "awefd_adjoint_release", "awefd_batch_run_ckpt", "awefd_batch_adjoint_grad",
-    "awefd_traces_dev", "awefd_batch_set_depth", "awefd_batch_tune_depth",
+    "awefd_traces_dev", "awefd_batch_set_depth", "awefd_batch_tune_depth", "awefd_run_noalloc",
 )
 
 
@@ -96,6 +96,7 @@ def load():
         "awefd_batch_set_depth": (C.c_int, [C.c_void_p, C.c_int]),
         "awefd_batch_tune_depth": (C.c_int, [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int), P(C.c_int), P(C.c_double)]),
         "awefd_batch_adjoint_grad": (C.c_int, [C.c_void_p, P(C.c_void_p), C.c_void_p]),
+        "awefd_run_noalloc": (C.c_int, [C.c_void_p, P(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)

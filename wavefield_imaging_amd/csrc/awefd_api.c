@@ -426,14 +426,16 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
     return rc;
 }
 
+static int setter_done(awefd_model *m);
+
 int awefd_set_model(awefd_model *m, const float *vel, const float *den)
 {
     if (!m || !vel) return awefd__fail(AWEFD_EINVAL, "awefd_set_model: NULL argument");
     HIPCHK(hipSetDevice(m->dev));
     int rc = is_cart(m) ? cart_set_model(m, vel, den) : awefd__spher_set_model(m, vel);
     if (rc == 0) m->model_set = 1;
-    m->gen++;
-    return rc;
+    const int rd = setter_done(m);
+    return rc ? rc : rd;
 }
 
 static int set_points(awefd_model *m, int which, int n, const float *coords)
@@ -594,8 +596,8 @@ int awefd_set_sources(awefd_model *m, int ns, const float *coords)
     if (!is_cart(m)) rc = awefd__spher_set_sources(m);
     else if (!m->inj_rec) rc = cart_build_inj(m);
     if (!rc) m->src_set = 1;
-    m->gen++;
-    return rc;
+    const int rd = setter_done(m);
+    return rc ? rc : rd;
 }
 
 int awefd_set_receivers(awefd_model *m, int nr, const float *coords)
@@ -608,9 +610,10 @@ int awefd_set_receivers(awefd_model *m, int nr, const float *coords)
     if (!rc && m->sinc)
         rc = upload(m, (void **)&m->d_rsinc, m->h_rsinc, sizeof(awefd_sinc_pt) * (nr > 0 ? nr : 1));
     if (!rc && m->inj_rec) rc = cart_build_inj(m);
+    if (!rc && !is_cart(m)) rc = awefd__spher_set_receivers(m);
     if (!rc) m->rec_set = 1;
-    m->gen++;
-    return rc;
+    const int rd = setter_done(m);
+    return rc ? rc : rd;
 }
 
 int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav)
@@ -620,7 +623,7 @@ int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav)
     int rc = upload(m, (void **)&m->d_wav, wav, sizeof(float) * nsamples);
     if (!rc && is_cart(m) && m->p.adj) {
         /* adj=y reads the rows back to front; the row length depends on the
-         * points set later, so the reversed copy is made at run time */
+         * points, so the reversed copy is made once the set-up is complete */
         free(m->h_wav);
         m->h_wav = malloc(sizeof(float) * nsamples);
         if (!m->h_wav) rc = awefd__fail(AWEFD_ENOMEM, "wavelet copy");
@@ -630,8 +633,8 @@ int awefd_set_wavelet(awefd_model *m, long nsamples, const float *wav)
         m->nwav = nsamples;
         m->wav_set = 1;
     }
-    m->gen++;
-    return rc;
+    const int rd = setter_done(m);
+    return rc ? rc : rd;
 }
 
 int awefd_debug_points(const awefd_model *m, int which, int *idx, float *w)
@@ -814,31 +817,58 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
     a->rsinc = m->sinc ? m->d_rsinc : NULL;
 }
 
-static int check_ready(awefd_model *m)
+/* samples of the wavelet the loop reads */
+static long wavelet_need(const awefd_model *m, long *row_out)
+{
+    const long row = m->p.expl ? 1 : m->inj_rec ? m->nr : m->ns;
+    if (row_out) *row_out = row;
+    return is_cart(m) ? (long)m->p.nt * row : m->p.nt;
+}
+
+/* adj=y: step it reads row nt-1-it (MAWEFDcart2Dcpu.c:305-316,
+ * MAWEFDcart3Dcpu.c:304-315); the kernels read row it of this copy.  The row
+ * length depends on the points, so the copy is made by the last setter that
+ * completes the set-up (and again by a run if it is stale). */
+static int build_wrev(awefd_model *m)
+{
+    long row;
+    const long need = wavelet_need(m, &row);
+    if (m->nwav < need) return AWEFD_OK;   /* the run reports the short wavelet */
+    float *r = malloc(sizeof(float) * (size_t)(need > 0 ? need : 1));
+    if (!r) return awefd__fail(AWEFD_ENOMEM, "reversed wavelet");
+    for (int it = 0; it < m->p.nt; it++)
+        memcpy(r + (long)it * row, m->h_wav + (long)(m->p.nt - 1 - it) * row, sizeof(float) * row);
+    free_dev((void **)&m->d_wavr);
+    int rc = upload(m, (void **)&m->d_wavr, r, sizeof(float) * (size_t)(need > 0 ? need : 1));
+    free(r);
+    if (rc) return rc;
+    m->wrev_gen = m->gen;
+    m->wrev_ok = 1;
+    return AWEFD_OK;
+}
+
+/* end of every setter: with the set-up complete, build what a run would
+ * otherwise build, so that runs allocate nothing */
+static int setter_done(awefd_model *m)
+{
+    m->gen++;
+    if (is_cart(m) && m->p.adj && m->model_set && m->src_set && m->rec_set && m->wav_set) return build_wrev(m);
+    return AWEFD_OK;
+}
+
+/* may_alloc = 0: report a stale derived table instead of rebuilding it */
+static int check_ready(awefd_model *m, int may_alloc)
 {
     if (!m->model_set) return awefd__fail(AWEFD_ESTATE, "velocity not set");
     if (!m->src_set) return awefd__fail(AWEFD_ESTATE, "sources not set");
     if (!m->rec_set) return awefd__fail(AWEFD_ESTATE, "receivers not set");
     if (!m->wav_set) return awefd__fail(AWEFD_ESTATE, "wavelet not set");
-    long need;
-    const long row = m->p.expl ? 1 : m->inj_rec ? m->nr : m->ns;
-    if (is_cart(m)) need = (long)m->p.nt * row;
-    else need = m->p.nt;
+    const long need = wavelet_need(m, NULL);
     if (m->nwav < need)
         return awefd__fail(AWEFD_EINVAL, "wavelet has %ld samples, the loop reads %ld", m->nwav, need);
     if (is_cart(m) && m->p.adj && (!m->wrev_ok || m->wrev_gen != m->gen)) {
-        /* adj=y: step it reads row nt-1-it (MAWEFDcart2Dcpu.c:305-316,
-         * MAWEFDcart3Dcpu.c:304-315); the kernels read row it of this copy */
-        float *r = malloc(sizeof(float) * (size_t)(need > 0 ? need : 1));
-        if (!r) return awefd__fail(AWEFD_ENOMEM, "reversed wavelet");
-        for (int it = 0; it < m->p.nt; it++)
-            memcpy(r + (long)it * row, m->h_wav + (long)(m->p.nt - 1 - it) * row, sizeof(float) * row);
-        free_dev((void **)&m->d_wavr);
-        int rc = upload(m, (void **)&m->d_wavr, r, sizeof(float) * (size_t)(need > 0 ? need : 1));
-        free(r);
-        if (rc) return rc;
-        m->wrev_gen = m->gen;
-        m->wrev_ok = 1;
+        if (!may_alloc) return awefd__fail(AWEFD_ESTATE, "adj=y: the reversed wavelet is not built");
+        return build_wrev(m);
     }
     return AWEFD_OK;
 }
@@ -949,7 +979,7 @@ static int cart_loop_graph(awefd_model *m, float *dtr)
 
 static int run_common(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, void *user)
 {
-    int rc = check_ready(m);
+    int rc = check_ready(m, 1);
     if (rc) return rc;
     HIPCHK(hipSetDevice(m->dev));
     if ((rc = awefd__ensure_traces(m))) return rc;
@@ -981,6 +1011,29 @@ int awefd_run(awefd_model *m, float *traces, int jsnap, awefd_snap_fn cb, void *
     if (traces && m->nr > 0 && !m->adj2)
         HIPCHK(hipMemcpy(traces, m->d_traces, sizeof(float) * (size_t)m->p.nt * m->nr,
                          hipMemcpyDeviceToHost));
+    return AWEFD_OK;
+}
+
+int awefd_run_noalloc(awefd_model *m, float *traces)
+{
+    if (!m) return awefd__fail(AWEFD_EINVAL, "NULL model");
+    if (m->prof) return awefd__fail(AWEFD_EINVAL, "awefd_run_noalloc: profiling is on (it allocates events)");
+    int rc = check_ready(m, 0);
+    if (rc) return rc;
+    const size_t nrow = m->adj2 ? 0 : (size_t)m->nr;
+    const size_t bytes = sizeof(float) * (size_t)m->p.nt * nrow;
+    if (bytes > AWEFD_NOALLOC_TRACE_BYTES)
+        return awefd__fail(AWEFD_EINVAL, "awefd_run_noalloc: %d steps x %zu receivers need %zu bytes of trace "
+                           "staging; the scratch holds %u", m->p.nt, nrow, bytes, AWEFD_NOALLOC_TRACE_BYTES);
+    HIPCHK(hipSetDevice(m->dev));
+    float *scr = NULL;
+    const int e = awefd__scratch(m->dev, &scr);
+    if (e) return awefd__fail(AWEFD_EHIP, "device scratch: %s", hipGetErrorString((hipError_t)e));
+    if ((rc = awefd_reset(m))) return rc;
+    rc = is_cart(m) ? cart_loop(m, scr, 0, NULL, NULL) : awefd__spher_loop(m, scr, 0, NULL, NULL);
+    if (rc) return rc;
+    if (traces && bytes) HIPCHK(hipMemcpyAsync(traces, scr, bytes, hipMemcpyDeviceToHost, m->stream));
+    HIPCHK(hipStreamSynchronize(m->stream));
     return AWEFD_OK;
 }
 

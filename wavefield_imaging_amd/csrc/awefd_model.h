@@ -121,6 +121,10 @@ void   awefd__adj_zero_jobs(const awefd_model *m, awefd_copy_job *jobs);
 int  awefd__spher_create(awefd_model *m);
 int  awefd__spher_set_model(awefd_model *m, const float *vel);
 int  awefd__spher_set_sources(awefd_model *m);
+int  awefd__spher_set_receivers(awefd_model *m);
+/* device address of the current device's fixed scratch (awefd_scratch.hip);
+ * returns a hipError_t */
+int  awefd__scratch(int dev, float **out);
 /* injection factor of source src: the expl=y reflectivity, else 1 (< 0: no model) */
 float awefd__spher_src_factor(const awefd_model *m, int src);
 int  awefd__spher_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, void *user);

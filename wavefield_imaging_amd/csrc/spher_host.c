@@ -40,12 +40,14 @@ typedef struct {
     float *d_iw;
     int ninode;
     int inj_dirty;
+    int *d_rrtp;                     /* receiver map in (r, theta, phi) order     */
     float *h_velp;                   /* padded raw velocity                       */
     long nsr, nst, nsp;
     float tdr, dr2, tdt, dt2, dp2;
 } spher_state;
 
 static spher_state *st(awefd_model *m) { return (spher_state *)m->spher; }
+static int build_injection(awefd_model *m);
 
 static void dfree(void *pp)
 {
@@ -78,6 +80,7 @@ void awefd__spher_free(awefd_model *m)
     dfree(&s->d_inode);
     dfree(&s->d_ibeg);
     dfree(&s->d_iw);
+    dfree(&s->d_rrtp);
     free(s->h_velp);
     free(s);
     m->spher = NULL;
@@ -287,13 +290,34 @@ int awefd__spher_set_model(awefd_model *m, const float *vel)
     free(s->h_velp);
     s->h_velp = vp;
     s->inj_dirty = 1;
+    if (!rc && m->src_set) rc = build_injection(m);
     return rc;
 }
 
+/* The injection list needs the sources and (expl=y) the padded velocity:
+ * built by whichever of the two setters comes last, so a run allocates
+ * nothing. */
 int awefd__spher_set_sources(awefd_model *m)
 {
-    st(m)->inj_dirty = 1;
-    return AWEFD_OK;
+    spher_state *s = st(m);
+    s->inj_dirty = 1;
+    return s->h_velp ? build_injection(m) : AWEFD_OK;
+}
+
+/* Receiver map in (r, theta, phi) order for the extraction kernel. */
+int awefd__spher_set_receivers(awefd_model *m)
+{
+    spher_state *s = st(m);
+    int *h = malloc(sizeof(int) * 3 * (m->nr > 0 ? m->nr : 1));
+    if (!h) return awefd__fail(AWEFD_ENOMEM, "receiver map");
+    for (int r = 0; r < m->nr; r++) {
+        h[3 * r] = m->h_rj[3 * r + 1];
+        h[3 * r + 1] = m->h_rj[3 * r];
+        h[3 * r + 2] = m->ndim == 3 ? m->h_rj[3 * r + 2] : 0;
+    }
+    int rc = up(&s->d_rrtp, h, sizeof(int) * 3 * (m->nr > 0 ? m->nr : 1));
+    free(h);
+    return rc;
 }
 
 /* Reflectivity |v0 - vk| / (v0 + vk) of the first neighbour k whose velocity
@@ -446,22 +470,6 @@ static void spher_args(const awefd_model *m, awefd_spher_args *a)
     a->rw = m->d_rw;
 }
 
-/* Receiver map in (r, theta, phi) order for the extraction kernel. */
-static int ensure_rec_rtp(awefd_model *m, int **d_rj)
-{
-    int *h = malloc(sizeof(int) * 3 * (m->nr > 0 ? m->nr : 1));
-    if (!h) return awefd__fail(AWEFD_ENOMEM, "receiver map");
-    for (int r = 0; r < m->nr; r++) {
-        h[3 * r] = m->h_rj[3 * r + 1];
-        h[3 * r + 1] = m->h_rj[3 * r];
-        h[3 * r + 2] = m->ndim == 3 ? m->h_rj[3 * r + 2] : 0;
-    }
-    *d_rj = NULL;
-    int rc = up(d_rj, h, sizeof(int) * 3 * (m->nr > 0 ? m->nr : 1));
-    free(h);
-    return rc;
-}
-
 /* The time loop (MAWEFDspher2Dcpu.c:264-317, MAWEFDspher3Dcpu.c:302-359).
  * m->dC holds the current field, m->dP the previous one. */
 int awefd__spher_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, void *user)
@@ -469,20 +477,16 @@ int awefd__spher_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, v
     spher_state *s = st(m);
     int rc;
     if (s->inj_dirty && (rc = build_injection(m))) return rc;
-    int *d_rj = NULL;
-    if ((rc = ensure_rec_rtp(m, &d_rj))) return rc;
+    if (!s->d_rrtp) return awefd__fail(AWEFD_ESTATE, "receivers not set");
     awefd_spher_args a;
     spher_args(m, &a);
-    a.rj = d_rj;
+    a.rj = s->d_rrtp;
     float *host = NULL;
     int npairs = 0;
     const int nt = m->p.nt;
     if (m->prof && m->nev < 2 * nt) {
         hipEvent_t *ev = realloc(m->ev, sizeof(hipEvent_t) * 2 * nt);
-        if (!ev) {
-            hipFree(d_rj);
-            return awefd__fail(AWEFD_ENOMEM, "event array");
-        }
+        if (!ev) return awefd__fail(AWEFD_ENOMEM, "event array");
         m->ev = ev;
         for (int i = m->nev; i < 2 * nt; i++) hipEventCreate(&m->ev[i]);
         m->nev = 2 * nt;
@@ -536,7 +540,6 @@ int awefd__spher_loop(awefd_model *m, float *dtr, int jsnap, awefd_snap_fn cb, v
         if (he != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "profiling events: %s", hipGetErrorString(he));
     }
     hipStreamSynchronize(m->stream);
-    hipFree(d_rj);
     return rc;
 }
 
