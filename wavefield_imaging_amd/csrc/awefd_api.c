@@ -130,8 +130,25 @@ static void cart_tiling(awefd_model *m)
     m->tz = 64;
     /* the z-high ABC recurrence needs its nop+2 nodes in the last tile */
     m->zsep = m->sponge && m->ndim == 3 && m->ntz > 1 && n1 - 64 * (m->ntz - 1) < m->nop + 2;
-    m->ntx = (n2 + AWEFD_CART_TX - 1) / AWEFD_CART_TX;
-    if (m->ndim == 3) {
+    const char *ce = getenv("AWEFD_COL");
+    m->col = m->ndim == 3 && !(ce && atoi(ce) == 0);
+    m->tx = m->col ? AWEFD_COL_RX : AWEFD_CART_TX;
+    m->ntx = (n2 + m->tx - 1) / m->tx;
+    if (m->col) {
+        /* one wave per (z tile, RX rows) column chunk of about
+         * AWEFD_COL_CHUNK planes (default 8) */
+        const char *ch = getenv("AWEFD_COL_CHUNK");
+        const long chunk = ch && atol(ch) > 0 ? atol(ch) : 8;
+        long nty = (n3 + chunk - 1) / chunk;
+        if (nty > n3) nty = n3;
+        if (nty < 1) nty = 1;
+        const long plane = (long)m->ld * n2 * 4;
+        while (nty < n3 && ((n3 + nty - 1) / nty + 2L * m->nop) * plane >= (1L << 30)) nty++;
+        /* the chunk's LDS tables (5 floats per plane) stay within 20 KB */
+        while (nty < n3 && (n3 + nty - 1) / nty > 1024) nty++;
+        m->nty = (int)nty;
+        m->ychunk = (int)((n3 + nty - 1) / nty);
+    } else if (m->ndim == 3) {
         int cols = m->ntz * m->ntx;
         int nty = (m->target_blocks + cols / 2) / cols;
         if (nty > n3) nty = n3;
@@ -147,9 +164,12 @@ static void cart_tiling(awefd_model *m)
         m->nty = 1;
     }
     m->nblk = m->ntz * m->ntx * m->nty;
+    /* the column kernel's blocks: 8 XCD shares of ceil(ntx / 8) x tiles */
+    if (m->col) m->nblk = 8 * m->ntz * ((m->ntx + 7) / 8) * m->nty;
     if (getenv("AWEFD_VERBOSE"))
-        fprintf(stderr, "awefd: step kernel tiles %d(z) x %d(x) x %d(y, <= %d planes) = %d blocks, "
-                "target %d\n", m->ntz, m->ntx, m->nty, m->ychunk, m->nblk, m->target_blocks);
+        fprintf(stderr, "awefd: step kernel%s tiles %d(z) x %d(x) x %d(y, <= %d planes) = %d blocks, "
+                "target %d\n", m->col ? " (columns)" : "", m->ntz, m->ntx, m->nty, m->ychunk, m->nblk,
+                m->target_blocks);
 }
 
 int awefd_create(const awefd_params *p, awefd_model **out)
@@ -579,7 +599,7 @@ static int cart_build_inj(awefd_model *m)
     awefd_inj *e;
     int *ibeg, nbucket;
     long n;
-    int rc = awefd__cart_buckets(m, m->tz, m->ntz, AWEFD_CART_TX, &e, &ibeg, &nbucket, &n);
+    int rc = awefd__cart_buckets(m, m->tz, m->ntz, m->tx, &e, &ibeg, &nbucket, &n);
     if (rc) return rc;
     rc = upload(m, (void **)&m->d_inj, e, sizeof(*e) * (n > 0 ? n : 1));
     if (!rc) rc = upload(m, (void **)&m->d_ibeg, ibeg, sizeof(int) * (nbucket + 1));
@@ -791,6 +811,7 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
     a->ntx = m->ntx;
     a->ychunk = m->ychunk;
     a->nty = m->nty;
+    a->col = m->col;
     a->nblk = m->nblk;
     a->dabc = m->sponge;
     a->nb = m->nb;

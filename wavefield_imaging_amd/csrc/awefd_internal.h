@@ -20,6 +20,12 @@ extern "C" {
 #define AWEFD_NWAVE 4             /* waves per block of the cartesian step kernel */
 #endif
 #define AWEFD_CART_TX (AWEFD_NWAVE * AWEFD_RX) /* x rows per stencil block     */
+#ifndef AWEFD_COL_RX
+#define AWEFD_COL_RX 2            /* x rows per wave of the 3-D column step kernel */
+#endif
+#ifndef AWEFD_COL_PD
+#define AWEFD_COL_PD 3            /* planes its loads run ahead                    */
+#endif
 #define AWEFD_BATCH_NWAVE 4      /* waves per block of the batched 2-D kernel     */
 #define AWEFD_BATCH_NWAVE3 8     /* waves per block of the three-step 2-D pass    */
 
@@ -68,6 +74,7 @@ typedef struct {
     float bz[AWEFD_MAX_NOP], bx[AWEFD_MAX_NOP], by[AWEFD_MAX_NOP];
     /* tiling */
     int tz, ntz, ntx, ychunk, nty, nblk; /* ychunk: longest y chunk      */
+    int col;                 /* 3-D: the column kernel (one wave per block)    */
     /* sponge: per-axis factors, 1 outside the strips */
     int dabc, nb;
     const float *sp1, *sp2, *sp3;

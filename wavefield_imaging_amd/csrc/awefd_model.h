@@ -61,6 +61,8 @@ struct awefd_model {
 
     /* cartesian tiling */
     int tz, ntz, ntx, ychunk, nty, nblk;
+    int tx;                       /* x rows per step-kernel tile (bucket width) */
+    int col;                      /* 3-D step in columns (k_cart_col)        */
     int target_blocks;            /* stencil blocks the y chunking aims at   */
     int zsep;                     /* z-high one-way ABC as a separate pass   */
 

@@ -493,6 +493,295 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
     }
 }
 
+/* ---- The 3-D step in columns ----
+ *
+ * A block is one wave: 64 z nodes (one per lane) of RX x rows, streaming a
+ * y range of the grid (the whole axis unless the grid is too small to fill
+ * the chip with columns).  Every per-plane input (previous field, (v dt)^2,
+ * the current field's head plane, its z- and x-halo values, density
+ * records) is loaded PD planes ahead into rotating register slots, so a
+ * wave keeps PD planes of loads in flight and no computation waits on a load
+ * issued in its own iteration.  The x-halo rows come from L2: the
+ * neighbouring columns stream the same planes at about the same time.  The
+ * y sponge factors and the z-face one-way coefficients of the block's planes
+ * are staged in LDS once.  No barrier in the plane loop; with whole-axis
+ * columns each current-field plane is read from HBM once (no chunk
+ * prologues).  The arithmetic, operation order and selections are the
+ * tiled body's (cart_step_body), so the fields are the same bit for bit. */
+
+/* ring slots of the current field: a multiple of PD + 1 holding planes
+ * y - NOP .. y + NOP + PD */
+template <int NOP, int PD>
+constexpr int col_ring()
+{
+    return ((2 * NOP + 1 + PD + PD) / (PD + 1)) * (PD + 1);
+}
+
+/* loads of plane y into stage slot S: previous field, (v dt)^2, the z-halo
+ * value of each row and the 2 NOP x-halo rows of the current field */
+template <int NOP, int RX, bool VD>
+__device__ __forceinline__ void col_stage(float (&p)[RX], float (&v)[RX], float (&zh)[RX], float (&hal)[2 * NOP],
+                                          float4 (&g)[RX], rsrc_t rP, rsrc_t rV, rsrc_t rC, rsrc_t rR, unsigned vz,
+                                          unsigned vzh, unsigned vz4, int x0, int y, int ybase, unsigned rowb,
+                                          unsigned psz)
+{
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const unsigned s = soff(x0 + j, y, ybase, rowb, psz);
+        p[j] = bld<NT>(rP, vz, s);
+        v[j] = bld<NT>(rV, vz, s);
+        zh[j] = bld(rC, vzh, s);
+        if (VD) g[j] = bld4(rR, vz4, 4u * s);
+    }
+#pragma unroll
+    for (int h = 0; h < 2 * NOP; h++) {
+        const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
+        hal[h] = bld(rC, vz, soff(hx, y, ybase, rowb, psz));
+    }
+}
+
+template <int NOP, bool SSE, bool VD, int RX, int PD>
+__device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *Pout, const float *Cin, int it,
+                                              float *trow, int bid, float *lds)
+{
+    constexpr int QR = col_ring<NOP, PD>();
+    constexpr int NS = PD + 1;
+    const int lane = threadIdx.x;
+    if (bid >= a.nblk) {
+        cart_extract<3>(a, Cin, trow, (bid - a.nblk) * WAVE + lane);
+        return;
+    }
+    /* Block b runs on XCD b % 8.  XCD k owns the columns of x tiles
+     * [ntx k / 8, ntx (k + 1) / 8) and takes them z tile fastest, then x,
+     * then y chunk: the columns that share x-halo rows, and the chunks that
+     * share prologue planes, run on one XCD close together in time, so those
+     * re-reads hit its L2, and the eight XCDs stream the same planes at a
+     * time.  Blocks past an XCD's share (when 8 does not divide ntx) exit. */
+    const int xcd = bid & 7, slot = bid >> 3;
+    const int xb = (int)((long)a.ntx * xcd / 8), nxk = (int)((long)a.ntx * (xcd + 1) / 8) - xb;
+    if (nxk == 0) return;
+    const int tzi = slot % a.ntz;
+    const int txi = xb + (slot / a.ntz) % nxk;
+    const int tyi = slot / (a.ntz * nxk);
+    if (tyi >= a.nty) return;
+    const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+    const int z0 = tzi * WAVE;
+    const int iz = z0 + lane;
+    const bool zin = iz < n1;
+    const int x0 = txi * RX;
+    const int y0 = (int)((long)tyi * n3 / a.nty);
+    const int y1 = (int)((long)(tyi + 1) * n3 / a.nty);
+    const long sy = (long)a.ld * n2;
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    const unsigned psz = (unsigned)(sy * 4);
+    const int ybase = max(y0 - NOP, 0);
+    const int ytop = min(y1 + NOP, n3);
+    const unsigned nbytes = (unsigned)(ytop - ybase) * psz;
+    const rsrc_t rC = mk_rsrc(Cin + ybase * sy, nbytes);
+    const rsrc_t rP = mk_rsrc(Pout + ybase * sy, nbytes);
+    const rsrc_t rV = mk_rsrc(a.vel + ybase * sy, nbytes);
+    const rsrc_t rR = mk_rsrc(VD ? (const void *)(a.rho + 4 * ybase * sy) : (const void *)a.vel,
+                              VD ? 4u * nbytes : 0u);
+    const unsigned vz = zin ? (unsigned)iz * 4u : OOB;
+    const unsigned vz4 = zin ? (unsigned)iz * 16u : OOB;
+    int zz = -1;
+    if (lane < NOP) zz = z0 - 1 - lane;
+    else if (lane >= WAVE - NOP) zz = z0 + WAVE + (WAVE - 1 - lane);
+    const unsigned vzh = (zz >= 0 && zz < n1) ? (unsigned)zz * 4u : OOB;
+
+    const bool zint = iz >= NOP && iz < n1 - NOP;
+    const bool zsp = a.dabc && zin && (iz < a.nb || iz >= n1 - a.nb);
+    const float spz = zsp ? a.sp1[iz] : 1.f;
+    const bool zsp_w = a.dabc && (z0 < a.nb || z0 + WAVE > n1 - a.nb);
+    const bool zdef = a.zsep && iz >= n1 - NOP - 2;
+    float spx[RX];
+    unsigned xint_m = 0, xdef_m = 0, xin_m = 0;
+    bool xsp_w = false;
+#pragma unroll
+    for (int j = 0; j < RX; j++) {
+        const int ix = x0 + j;
+        const bool xsp = a.dabc && ix < n2 && (ix < a.nb || ix >= n2 - a.nb);
+        xsp_w = xsp_w || xsp;
+        spx[j] = xsp ? a.sp2[ix] : 1.f;
+        xin_m |= ix < n2 ? 1u << j : 0u;
+        xint_m |= (ix >= NOP && ix < n2 - NOP) ? 1u << j : 0u;
+        xdef_m |= (a.abc && (ix <= NOP + 1 || ix >= n2 - NOP - 2)) ? 1u << j : 0u;
+    }
+    const bool sp_xz = zsp_w || xsp_w;
+    const bool zlow = a.abc && tzi == 0 && !a.fsrf;
+    const bool zhigh = a.abc && !a.zsep && tzi == a.ntz - 1;
+
+    /* LDS: the y sponge factor of each plane of the range, then (z-edge
+     * columns) the z-low and z-high one-way coefficients of each row */
+    const int ny = y1 - y0;
+    float *ysp_l = lds;
+    float *zb_l = lds + ny;
+    for (int i = lane; i < ny; i += WAVE) ysp_l[i] = a.dabc ? a.sp3[y0 + i] : 1.f;
+    if (zlow || zhigh)
+        for (int i = lane; i < ny * RX; i += WAVE) {
+            const long o = (long)(y0 + i / RX) * n2 + min(x0 + i % RX, n2 - 1);
+            zb_l[2 * i] = zlow ? a.bzl[o] : 0.f;
+            zb_l[2 * i + 1] = zhigh ? a.bzh[o] : 0.f;
+        }
+    __syncthreads();
+
+    /* injection entries of this column, from plane y0 on */
+    const int col = tzi + a.ntz * txi;
+    int ib = a.ibeg[col];
+    const int ie = a.ibeg[col + 1];
+    while (ib < ie && a.inj[ib].y < y0) ib++;
+
+    /* current field ring (plane p in slot (p - y0 + NOP) % QR) and stage
+     * slots (plane y in slot (y - y0) % NS) */
+    float q[RX][QR];
+    float sp_[NS][RX], sv_[NS][RX], sz_[NS][RX], sh_[NS][2 * NOP];
+    float4 sg_[NS][RX];
+#pragma unroll
+    for (int k = 0; k < 2 * NOP + PD; k++)
+#pragma unroll
+        for (int j = 0; j < RX; j++) q[j][k] = bld(rC, vz, soff(x0 + j, y0 - NOP + k, ybase, rowb, psz));
+#pragma unroll
+    for (int k = 0; k < PD; k++)
+        col_stage<NOP, RX, VD>(sp_[k], sv_[k], sz_[k], sh_[k], sg_[k], rP, rV, rC, rR, vz, vzh, vz4, x0, y0 + k,
+                               ybase, rowb, psz);
+
+    for (int yg = y0; yg < y1; yg += QR) {
+#pragma unroll
+        for (int R = 0; R < QR; R++) {
+            const int y = yg + R;
+            if (y >= y1) break;
+            const int B = R % NS;                         /* stage slot of plane y */
+            const int C0 = (R + NOP) % QR;                /* ring slot of plane y  */
+            if (y + PD < y1) {
+                const int Bn = (R + PD) % NS;
+#pragma unroll
+                for (int j = 0; j < RX; j++)
+                    q[j][(R + 2 * NOP + PD) % QR] = bld(rC, vz, soff(x0 + j, y + NOP + PD, ybase, rowb, psz));
+                col_stage<NOP, RX, VD>(sp_[Bn], sv_[Bn], sz_[Bn], sh_[Bn], sg_[Bn], rP, rV, rC, rR, vz, vzh, vz4, x0,
+                                       y + PD, ybase, rowb, psz);
+            }
+            float cen[RX];
+#pragma unroll
+            for (int j = 0; j < RX; j++) cen[j] = q[j][C0];
+            const bool yint = y >= NOP && y < n3 - NOP;
+            const bool ysp = a.dabc && (y < a.nb || y >= n3 - a.nb);
+            const float spy = ysp ? ysp_l[y - y0] : 1.f;
+            const bool ysh = a.abc && (y <= NOP + 1 || y >= n3 - NOP - 2);
+
+            float nv[RX], zp1[RX], zm1[RX];
+#pragma unroll
+            for (int j = 0; j < RX; j++) {
+                float zm[NOP], zp[NOP];
+                float hlo = sz_[B][j], hhi = sz_[B][j];
+                zm[0] = shr1(hlo, cen[j]);
+                zp[0] = shl1(hhi, cen[j]);
+#pragma unroll
+                for (int k = 1; k < NOP; k++) {
+                    hlo = shl1(hlo, hlo);
+                    hhi = shr1(hhi, hhi);
+                    zm[k] = shr1(hlo, zm[k - 1]);
+                    zp[k] = shl1(hhi, zp[k - 1]);
+                }
+                zp1[j] = zp[0];
+                zm1[j] = zm[0];
+                float p = sp_[B][j];
+                if (sp_xz || ysp) {
+                    p = p * spz;
+                    p = p * spx[j];
+                    p = p * spy;
+                }
+                const float c = cen[j];
+                float lap = c * a.c0, v;
+                if (SSE) {
+#pragma unroll
+                    for (int k = 1; k <= NOP; k++) {
+                        const float xpk = xnb<NOP, RX>(cen, sh_[B], j, k), xmk = xnb<NOP, RX>(cen, sh_[B], j, -k);
+                        lap = lap + a.cz[k - 1] * (zp[k - 1] + zm[k - 1]);
+                        lap = lap + a.cx[k - 1] * (xpk + xmk);
+                        lap = lap + a.cy[k - 1] * (q[j][(C0 + k) % QR] + q[j][(C0 + QR - k) % QR]);
+                    }
+                    if (VD) lap = den_term<NOP, 3, SSE>(lap, zp, zm, cen, sh_[B], q[j], (C0 + QR - NOP) % QR, j,
+                                                       sg_[B][j], a);
+                    const float t = 2.f * c - p;
+                    v = t + sv_[B][j] * lap;
+                } else {
+#pragma unroll
+                    for (int k = 1; k <= NOP; k++) {
+                        const float xpk = xnb<NOP, RX>(cen, sh_[B], j, k), xmk = xnb<NOP, RX>(cen, sh_[B], j, -k);
+                        float t = (zm[k - 1] + zp[k - 1]) * a.cz[k - 1] + (xmk + xpk) * a.cx[k - 1];
+                        t = t + (q[j][(C0 + QR - k) % QR] + q[j][(C0 + k) % QR]) * a.cy[k - 1];
+                        lap = lap + t;
+                    }
+                    if (VD) lap = den_term<NOP, 3, SSE>(lap, zp, zm, cen, sh_[B], q[j], (C0 + QR - NOP) % QR, j,
+                                                       sg_[B][j], a);
+                    const float vl = sv_[B][j] * lap;
+                    v = (float)(2.0 * (double)c - (double)p + (double)vl);
+                }
+                nv[j] = (zint && ((xint_m >> j) & 1u) && yint) ? v : p;
+            }
+
+            int e = ib;
+            for (; e < ie && a.inj[e].y == y; e++) {
+                const awefd_inj en = a.inj[e];
+                const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
+#pragma unroll
+                for (int j = 0; j < RX; j++) {
+                    const float inj = nv[j] + amp * en.w * en.w2 * en.w3 * sv_[B][j];
+                    nv[j] = (zin && en.z == iz && en.x == x0 + j) ? inj : nv[j];
+                }
+            }
+            ib = e;
+
+            if (zlow || zhigh) {
+#pragma unroll
+                for (int j = 0; j < RX; j++) {
+                    if (!((xin_m >> j) & 1u)) continue;
+                    if (zlow) {
+                        const float b = zb_l[2 * ((y - y0) * RX + j)];
+#pragma unroll
+                        for (int k = 0; k < NOP; k++) {
+                            const int zt = NOP - k;
+                            const float up = shl1(nv[j], nv[j]);
+                            const float w = zp1[j] + (cen[j] - up) * b;
+                            nv[j] = iz == zt ? w : nv[j];
+                        }
+                    }
+                    if (zhigh) {
+                        const float b = zb_l[2 * ((y - y0) * RX + j) + 1];
+#pragma unroll
+                        for (int k = 0; k < NOP; k++) {
+                            const int zt = n1 - NOP + k - 1;
+                            const float dn = shr1(nv[j], nv[j]);
+                            const float w = zm1[j] + (cen[j] - dn) * b;
+                            nv[j] = iz == zt ? w : nv[j];
+                        }
+                    }
+                }
+            }
+
+#pragma unroll
+            for (int j = 0; j < RX; j++) {
+                if (!((xin_m >> j) & 1u)) continue;
+                float v = nv[j];
+                if ((sp_xz || ysp) && !(((xdef_m >> j) & 1u) || ysh)) {
+                    float w = v * spz;
+                    w = w * spx[j];
+                    w = w * spy;
+                    v = zdef ? v : w;
+                }
+                bst<NT>(rP, vz, soff(x0 + j, y, ybase, rowb, psz), v);
+            }
+        }
+    }
+}
+
+template <int NOP, bool SSE, bool VD>
+__global__ __launch_bounds__(WAVE) void k_cart_col(const awefd_cart_args a)
+{
+    extern __shared__ float col_lds[];
+    cart_col_body<NOP, SSE, VD, AWEFD_COL_RX, AWEFD_COL_PD>(a, a.P, a.C, a.it, a.trow, blockIdx.x, col_lds);
+}
+
 /* The single-grid step kernel: one launch per time step of one model. */
 template <int NOP, int NDIM, bool SSE, bool VD>
 __global__ __launch_bounds__(NTHR0) void k_cart_step(const awefd_cart_args a)
@@ -1138,6 +1427,17 @@ __global__ void k_fill(float *p, long n, float v)
 template <int NOP, int NDIM, bool SSE>
 int launch_step(const awefd_cart_args *a, bool vd, hipStream_t s)
 {
+    if (NDIM == 3 && a->col) {
+        /* LDS: the y sponge factor and two z-face coefficients per row of
+         * each plane of the longest chunk */
+        const size_t lds = sizeof(float) * (size_t)a->ychunk * (1 + 2 * AWEFD_COL_RX);
+        const int nx = (a->nr + WAVE - 1) / WAVE;
+        if (vd)
+            hipLaunchKernelGGL((k_cart_col<NOP, SSE, true>), dim3(a->nblk + nx), dim3(WAVE), lds, s, *a);
+        else
+            hipLaunchKernelGGL((k_cart_col<NOP, SSE, false>), dim3(a->nblk + nx), dim3(WAVE), lds, s, *a);
+        return (int)hipGetLastError();
+    }
     const int nextract = (a->nr + NTHR0 - 1) / NTHR0;
     if (vd)
         hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, true>), dim3(a->nblk + nextract), dim3(NTHR0), 0, s, *a);
