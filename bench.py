@@ -9,12 +9,11 @@ sponge + one-way ABC (dabc=y), FP32.
 
 The default run also measures, under "also":
   cart2d    400 x 800 (z, x), nb = 50 (padded 500 x 900), nt = 3000, one
-            source, 800 receivers; with the CPU restatement over the whole run
+            source, 800 receivers
   spher3d   the spherical shell, 400 x 512 x 512 (r, theta, phi), r from
             3387.5 km, dr = 5 m, r*dtheta = 5 m at mid radius, 30-node sponge
             (padded 460 x 572 x 572 = 150,504,640 nodes), nt = 2000, one source
-            10 nodes below the top, 256 x 256 receivers (also selectable as the
-            headline with --workload spher3d)
+            10 nodes below the top, 256 x 256 receivers
   batch2d   256 independent 2-D shots of uneven size, batched graph vs the
             per-shot loop, in shots/s
   adjoint2d 64 shots on a 1000 x 2000 grid, nt = 4000, checkpoints every 64
@@ -22,16 +21,27 @@ The default run also measures, under "also":
             loop
 
 One benchmark step = one complete shot (all nt time steps) with every input
-already resident in HBM.  value = node-updates per second over the timed
-steps (Gcell-updates/s).  The roofline figure is for the dominant kernel
-(the fused step kernel): 16 algorithmic bytes per node-update (read the
-previous field, the current field and the velocity term, write the new
-field), times the padded nodes per launch, over the kernel's mean duration
-measured with HIP events on the stream it runs on.
+already resident in HBM; ms_per_step is therefore milliseconds per shot and
+ms_per_time_step the same per time step.  value = node-updates per second
+over the timed shots (Gcell-updates/s).
 
-cpu_baseline: the C oracle (a restatement of the reference CPU program,
-OpenMP over the host cores) on the same padded grid for a bounded number of
-time steps.
+Roofline: algorithmic bytes (16 per node update: read the previous field, the
+current field and the velocity term, write the new field) over the dominant
+kernel's time.  3-D: the kernel's mean duration from HIP events around every
+launch on its own stream, inside the timed region.  cart2d runs as one
+replayed graph: its time per step (kernel plus the graph's launch gaps, an
+upper bound on the kernel) over the timed region.  batch2d and adjoint2d run
+as graphs too: real bytes of their passes over the timed graphs (20 bytes per
+node for a three-step pass; 60 per node and step of a gradient: forward,
+recompute and the 28-byte adjoint step).
+
+cpu_baseline: the reference CPU path restated in C (oracle/liborc_ref.so:
+the 4-wide SSE step with schedule(static,1) at -O3, OpenMP over r for the
+spherical solve; bit-identical to the checker build), timed on the host cores
+on a bounded sample of the same workload.
+
+--calibrate-hbm MIB runs only a plain device copy of MIB MiB (for scaling the
+rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exits.
 """
 import argparse
 import json
@@ -50,7 +60,7 @@ WORKLOADS = {
                    traffic=None,
                    desc="cart2d 400x800 (z,x) h=5m nb=50 nt=%d dabc=y fdorder=4, padded 500x900, "
                         "1 source, 800 receivers"),
-    "cart3d": dict(kernel="k_cart_step<2,3,true>",
+    "cart3d": dict(kernel="k_cart_col<2,true,false>",
                    traffic="cart3d_step_traffic.json",
                    desc="cart3d 256x512x512 (z,x,y) h=5m nb=40 nt=%d dabc=y fdorder=4, "
                         "padded 336x592x592, 1 source, 128x128 receivers"),
@@ -62,7 +72,8 @@ WORKLOADS = {
 
 
 def _traffic(name):
-    """HBM bytes per launch of the step kernel from the committed PMC summary."""
+    """Calibrated HBM bytes per launch of the step kernel from the committed
+    counter summary."""
     if not name:
         return None
     path = os.path.join(ROOT, "profiles", name)
@@ -75,26 +86,83 @@ def _traffic(name):
         return None
 
 
-def cpu_baseline(case, steps):
-    """Time the C oracle (reference CPU path restated) on `steps` time steps."""
-    import ctypes as C
-
-    import numpy as np
-
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    return O
 
-    par = dict(case["par"], nt=steps)
-    O.run(case["kind"], par, case["vel"], case["sou"], case["rec"], case["wav"][:steps], final=False)
-    f = O.lib().orc_last_loop_seconds
+
+def _loop_seconds(O):
+    import ctypes as C
+    f = O.lib(True).orc_last_loop_seconds
     f.restype = C.c_double
-    secs = f()
+    return f()
+
+
+def _cores(O):
+    return int(O.lib(True).orc_threads())
+
+
+def cpu_baseline(case, steps):
+    """The reference-form CPU loop on `steps` time steps of `case`."""
+    import numpy as np
+
+    O = _oracle()
+    par = dict(case["par"], nt=steps)
+    O.run(case["kind"], par, case["vel"], case["sou"], case["rec"], case["wav"][:steps], final=False,
+          ref_form=True)
+    secs = _loop_seconds(O)
     cells = int(np.prod([n + 2 * par["nb"] for n in par["n"]]))
-    return dict(value=round(cells * steps / secs / 1e9, 4), unit="Gcell-updates/s",
-                cores=int(O.lib().orc_threads()), kind="port",
-                sample="%s padded %s, %d of %d time steps, %.1f s loop wall time"
+    return dict(value=round(cells * steps / secs / 1e9, 4), unit="Gcell-updates/s", cores=_cores(O), kind="port",
+                sample="%s padded %s, %d of %d time steps, %.1f s loop wall time (SSE step, -O3)"
                        % (case["kind"], "x".join(str(n + 2 * par["nb"]) for n in par["n"]),
                           steps, case["par"]["nt"], secs))
+
+
+def cpu_baseline_batch(bat, nshot_sample):
+    """The per-shot loop of the reference-form 2-D program over the first
+    shots of the batch, full length."""
+    import numpy as np
+
+    O = _oracle()
+    secs, cells = 0.0, 0
+    for sh in bat["shots"][:nshot_sample]:
+        O.run("cart2d", sh["par"], sh["vel"], sh["sou"], sh["rec"], bat["wav"], final=False, ref_form=True)
+        secs += _loop_seconds(O)
+        cells += int(np.prod([n + 2 * sh["par"]["nb"] for n in sh["par"]["n"]]))
+    nt = bat["shots"][0]["par"]["nt"]
+    return dict(value=round(nshot_sample / secs, 4), unit="shots/s", cores=_cores(O), kind="port",
+                gcells_per_s=round(cells * nt / secs / 1e9, 4),
+                sample="first %d of %d shots, all %d steps each, %.1f s of loop wall time (SSE step, -O3)"
+                       % (nshot_sample, len(bat["shots"]), nt, secs))
+
+
+def cpu_baseline_grad(case, fwd_steps=200, grad_steps=100):
+    """The reference has no gradient.  Timed: one shot's reference-form
+    forward loop (the reference's CPU path) and the FP64 CPU gradient of the
+    same scheme (forward, adjoint and gradient), each on a truncated run of
+    the full grid, scaled to nt; value = gradients/s of the FP64 CPU
+    gradient."""
+    import numpy as np
+
+    O = _oracle()
+    par, nt = case["par"], case["par"]["nt"]
+    sou, rec = case["shots"][0], case["rec"]
+    p1 = dict(par, nt=fwd_steps)
+    O.run("cart2d", p1, case["start"], sou, rec, case["wav"][:fwd_steps], final=False, ref_form=True)
+    fwd_s = _loop_seconds(O) * nt / fwd_steps
+    p2 = dict(par, nt=grad_steps)
+    t0 = time.perf_counter()
+    O.grad_f64(p2, case["start"], sou, rec, case["wav"][:grad_steps])
+    g_s = (time.perf_counter() - t0) * nt / grad_steps
+    cells = int(np.prod([n + 2 * par["nb"] for n in par["n"]]))
+    return dict(value=round(1.0 / g_s, 6), unit="gradients/s", cores=_cores(O), kind="port",
+                forward_shots_per_s=round(1.0 / fwd_s, 5),
+                forward_gcells_per_s=round(cells * nt / fwd_s / 1e9, 4),
+                sample="one shot on the full %s padded grid: the reference-form forward loop over %d of %d steps "
+                       "and the FP64 gradient over %d steps, both on %d threads and scaled to %d steps"
+                       % ("x".join(str(n + 2 * par["nb"]) for n in par["n"]), fwd_steps, nt, grad_steps,
+                          _cores(O), nt))
 
 
 def measure(workload, nt, steps, warmup):
@@ -117,38 +185,42 @@ def measure(workload, nt, steps, warmup):
         m.run_dev()
     # 3-D: the kernel's own events bracket every launch inside the timed
     # region.  2-D: the loop runs as one captured graph (a step is a few
-    # microseconds, launch overhead would dominate), so the timed region runs
-    # the graph and one more profiled (per-launch) run gives the kernel time.
+    # microseconds, launch overhead would dominate); its time per step over
+    # the timed region bounds the kernel's.
     eager_timed = m.ndim == 3
     m.profile(eager_timed)
     t0 = time.perf_counter()
     for _ in range(steps):
         m.run_dev()
     t1 = time.perf_counter()
-    if not eager_timed:
-        m.profile(True)
-        m.run_dev()
-    launches, kern_ms = m.kernel_stats()
+    secs = t1 - t0
+    if eager_timed:
+        launches, kern_ms = m.kernel_stats()
+        avg_ms = kern_ms / max(launches, 1)
+        timing = "HIP events around every launch of the timed shots"
+    else:
+        launches = nt * steps
+        avg_ms = secs / launches * 1e3
+        timing = "replayed graph: time per step over the timed shots (kernel plus in-graph launch gaps)"
     m.profile(False)
     m.close()
-    secs = t1 - t0
-    avg_ms = kern_ms / max(launches, 1)
     achieved = BYTES_PER_UPDATE * cells / (avg_ms * 1e-3) / 1e9
     out = dict(workload=w["desc"] % nt, cells_per_step=cells, time_steps_per_shot=nt,
                value=round(cells * nt * steps / secs / 1e9, 3),
                ms_per_step=round(secs / steps * 1e3, 3),
+               ms_per_time_step=round(secs / steps / nt * 1e3, 5),
                roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                          "traffic": _traffic(w["traffic"]), "kernel": w["kernel"],
-                         "kernel_mean_ms": round(avg_ms, 4), "launches": launches})
+                         "kernel_mean_ms": round(avg_ms, 5), "launches": launches, "timing": timing})
     return out, case
 
 
-def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
+def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True, fixed=None, cpu_shots=0):
     """256 independent 2-D shots of uneven size (nz ~ U[200,600], nx ~
     U[400,1200], nb=50): the batched graph-captured run (autotuned tile,
-    shot grouping and steps per launch) and the plain per-shot loop of the
-    single-grid kernel, in shots/s."""
+    shot grouping and steps per launch, or `fixed` = (rx, group_mb, depth))
+    and the plain per-shot loop of the single-grid kernel, in shots/s."""
     import numpy as np
 
     from wavefield_imaging_amd import awefd, synth
@@ -165,17 +237,39 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
     cells = int(sum(np.prod(m.padded) for m in ms))
     print("bench: batch2d %d shots, %.3g padded cells, nt=%d" % (nshot, cells, nt), file=sys.stderr, flush=True)
     b = awefd.Batch(ms)
-    rx, grp, depth, tune_ms = b.tune_depth(steps=tune_steps)
+    if fixed:
+        rx, grp, depth = fixed
+        b.set_tile(rx, grp)
+        b.set_depth(depth)
+        tune_ms = {}
+    else:
+        rx, grp, depth, tune_ms = b.tune_depth(steps=tune_steps)
     b.run()                                   # warm-up (graph capture + launch)
     t0 = time.perf_counter()
     for _ in range(steps):
         b.run()
     t1 = time.perf_counter()
-    batched = nshot * steps / (t1 - t0)
+    secs = (t1 - t0) / steps
+    batched = nshot / secs
+    # bytes of the run: passes of `depth` steps read the two input fields and
+    # V and write the last two fields (20 B/node; one step: 16 B/node)
+    npass, nsingle = (nt - 1) // depth if depth > 1 else 0, nt - ((nt - 1) // depth) * depth if depth > 1 else nt
+    bytes_run = (npass * 20 + nsingle * 16) * cells
+    achieved = bytes_run / secs / 1e9
+    kern = {1: "k_cart_batch2<2,true,%d,4>", 2: "k_cart_batch_t2<2,true,%d,4>",
+            3: "k_cart_batch_t3<2,true,%d,8>"}[depth] % rx
     out = dict(workload="batch2d %d shots nz~U[200,600] nx~U[400,1200] nb=50 nt=%d, %d padded cells"
                         % (nshot, nt, cells),
-               shots_per_s=round(batched, 3), gcells_per_s=round(cells * nt * steps / (t1 - t0) / 1e9, 3),
-               s_per_batch=round((t1 - t0) / steps, 4), tile_rx=rx, group_mb=grp, steps_per_launch=depth,
+               shots_per_s=round(batched, 3), gcells_per_s=round(cells * nt / secs / 1e9, 3),
+               s_per_batch=round(secs, 4), ms_per_time_step=round(secs / nt * 1e3, 5), tile_rx=rx, group_mb=grp,
+               steps_per_launch=depth,
+               roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kern,
+                         "bytes_per_run": bytes_run,
+                         "timing": "the timed graph: %d passes of %d steps at 20 B/node and %d single steps at "
+                                   "16 B/node over the run's wall time (fields of one shot fit the 256 MB "
+                                   "Infinity Cache, so HBM is not the only level serving them)"
+                                   % (npass, depth, nsingle)},
                tune_ms_per_step={"d%d_rx%d_g%d" % k: round(v, 4) for k, v in tune_ms.items()})
     b.close()
     if per_shot:
@@ -193,10 +287,12 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True):
         out.update(per_shot_loop_shots_per_s=round(loop, 3), speedup_vs_per_shot_loop=round(batched / loop, 2))
     for m in ms:
         m.close()
+    if cpu_shots:
+        out["cpu_baseline"] = cpu_baseline_batch(bat, cpu_shots)
     return out
 
 
-def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0):
+def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cpu=False):
     """Misfit gradients w.r.t. squared slowness, 1000 x 2000 grid, nb=50,
     nt=4000, checkpoints every 64 steps, 2000 receivers per shot: the per-shot
     loop (forward with checkpoints, residual, adjoint) timed on `sample`
@@ -254,6 +350,7 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     loop = sample / (t1 - t0)
+    bytes_grad = 60 * cells * nt     # forward 16 + recompute 16 + adjoint 28 B per node and step
     out = dict(workload="adjoint2d 1000x2000 nb=50 nt=%d ckpt=%d, %d shots x %d receivers, padded %s"
                         % (nt, K, nshot, nr, "x".join(map(str, m.padded[:2]))),
                per_shot_loop_gradients_per_s=round(loop, 3),
@@ -290,14 +387,39 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         bat = nshot / (t1 - t0)
+        achieved = bytes_grad * nshot / (t1 - t0) / 1e9
         out.update(batched_gradients_per_s=round(bat, 3), batched_s_per_gradient_sum=round(t1 - t0, 3),
+                   ms_per_time_step=round((t1 - t0) / nshot / nt * 1e3, 5),
                    batched_tile_rx=rx, batched_group_mb=group_mb,
                    batched_gcells_per_s_per_propagation=round(3 * cells * nt * nshot / (t1 - t0) / 1e9, 3),
-                   speedup_vs_per_shot_loop=round(bat / loop, 2))
+                   speedup_vs_per_shot_loop=round(bat / loop, 2),
+                   roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                             "kernel": "k_adj2d_batch<2> with k_cart_batch2 (forward, recompute)",
+                             "bytes_per_gradient": bytes_grad,
+                             "timing": "the timed batched gradient (forward graph, residuals, backward graph): "
+                                       "60 B per node and step (forward 16, recompute 16, adjoint 28) over its "
+                                       "wall time"})
         b.close()
         for mm in ms:
             mm.close()
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline_grad(case)
     return out
+
+
+def calibrate_hbm(mib, reps=5):
+    """A plain device copy of `mib` MiB, `reps` times: its FETCH_SIZE and
+    WRITE_SIZE per dispatch scale the same counters read on the kernels."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_hbm_mib": mib, "reps": reps}), flush=True)
 
 
 def main():
@@ -310,13 +432,18 @@ def main():
     ap.add_argument("--also", default="cart2d,spher3d,batch2d,adjoint2d",
                     help="comma-separated extra workloads measured after the headline one "
                          "(one shot each; empty for none)")
-    ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the CPU baseline")
+    ap.add_argument("--cpu-steps", type=int, default=24, help="time steps of the headline CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-hbm", type=int, metavar="MIB", default=0,
+                    help="only run a plain MIB-MiB device copy (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
               "(the time loop runs on a single GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_hbm:
+        calibrate_hbm(args.calibrate_hbm)
+        return
 
     res, case = measure(args.workload, args.nt, args.steps, args.warmup)
     out = {
@@ -327,13 +454,15 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": res["ms_per_step"],
+        "ms_per_time_step": res["ms_per_time_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded three-layer rough-base model, Ricker 2 MHz)",
         "config": {"workload": res["workload"], "cells_per_step": res["cells_per_step"],
-                   "time_steps_per_shot": res["time_steps_per_shot"]},
+                   "time_steps_per_shot": res["time_steps_per_shot"],
+                   "step": "one shot (all time steps)"},
         "roofline": res["roofline"],
     }
     if not args.no_cpu:
@@ -346,9 +475,9 @@ def main():
     for wl in [x for x in args.also.split(",") if x and x != args.workload]:
         try:
             if wl == "batch2d":
-                also.append(measure_batch(args.nt))
+                also.append(measure_batch(args.nt, cpu_shots=0 if args.no_cpu else 4))
             elif wl == "adjoint2d":
-                also.append(measure_grad())
+                also.append(measure_grad(cpu=not args.no_cpu))
             elif wl == "cart2d":
                 # the 2-D forward at its own length (nt=3000), with the CPU
                 # restatement timed over the whole run beside it
@@ -357,10 +486,13 @@ def main():
                     r["cpu_baseline"] = cpu_baseline(c2, 3000)
                 also.append(r)
             else:
-                r, _ = measure(wl, args.nt, 1, 1)
+                r, c3 = measure(wl, args.nt, 1, 1)
+                if not args.no_cpu:
+                    r["cpu_baseline"] = cpu_baseline(c3, 20)
                 also.append(r)
         except Exception as e:
             also.append({"workload": wl, "error": str(e)})
+        print("bench: %s done" % wl, file=sys.stderr, flush=True)
     if also:
         out["also"] = also
     print(json.dumps(out), flush=True)
