@@ -5,10 +5,13 @@ reference; its distance from the oracle's FP64 run of the same scheme is the
 rounding error the whole-loop FP32 arithmetic carries.  Relative L2 of the
 traces and of the final padded field, printed per workload:
 
-  cart2d   the 2-D forward at full size (400 x 800, nb 50, nt 3000)
-  cart3d   the 3-D forward at a quarter of each axis (64 x 128 x 128, nt 600)
-  spher3d  the spherical shell at 0.15 of each axis (60 x 76 x 76, nt 300)
-  batch2d  one shot of the batched workload at full size (nt 2000)
+  cart2d        the 2-D forward at full size (400 x 800, nb 50, nt 3000)
+  cart3d_full   the measured 3-D grid (256 x 512 x 512, nb 40), first 300 steps
+  spher3d_full  the measured spherical shell (400 x 512 x 512, nb 30), first 150
+                steps
+  cart3d        the 3-D forward at a quarter of each axis (64 x 128 x 128, nt 600)
+  spher3d       the spherical shell at 0.15 of each axis (60 x 76 x 76, nt 300)
+  batch2d       one shot of the batched workload at full size (nt 2000)
 
 (The gradient's FP64 error is checked in test_gpu_adjoint.py.)  The numbers
 are reported, not targets: FP32 leapfrog rounding grows with the number of
@@ -32,13 +35,15 @@ def _cases():
     shot = dict(b["shots"][0], wav=b["wav"])
     return [
         ("cart2d", lambda: synth.cart2d()),
+        ("cart3d_full", lambda: synth.cart3d(nt=300)),
+        ("spher3d_full", lambda: synth.spher3d(nt=150)),
         ("cart3d", lambda: synth.cart3d(scale=0.25, nt=600)),
         ("spher3d", lambda: synth.spher3d(scale=0.15, nt=300)),
         ("batch2d", lambda: shot),
     ]
 
 
-@pytest.mark.parametrize("name", ["cart2d", "cart3d", "spher3d", "batch2d"])
+@pytest.mark.parametrize("name", ["cart2d", "cart3d_full", "spher3d_full", "cart3d", "spher3d", "batch2d"])
 def test_fp64_error(orc, name):
     case = dict(_cases())[name]()
     kind = case["kind"]
