@@ -181,6 +181,9 @@ int awefd_launch_copy_jobs(const awefd_copy_job *jobs, const long *cbeg, int njo
 #define AWEFD_SPHER_NWAVE 4
 #endif
 #define AWEFD_SPHER_TX (AWEFD_SPHER_NWAVE * AWEFD_SPHER_RX)
+#ifndef AWEFD_SPHER_CRX
+#define AWEFD_SPHER_CRX 2         /* theta rows per wave of the 3-D column kernel */
+#endif
 
 typedef struct {
     float       *P;          /* current field (after this step's injection)   */
@@ -204,8 +207,8 @@ typedef struct {
     /* one-way tables: theta faces [phi][r], r faces [phi][theta] (the table
      * each rule uses), phi faces [theta][r] */
     const float *bthl, *bthh, *brl, *brh, *bphl, *bphh;
-    /* step tiling */
-    int ntz, ntx, nty, nblk;
+    /* step tiling; col: the 3-D column kernel (one wave per block) */
+    int ntz, ntx, nty, nblk, col;
     /* shell enumeration: counts of the r-face, theta-face and phi-face groups */
     long nsr, nst, nsp;
     /* injection: nodes (flat offsets) with their contributions in order */

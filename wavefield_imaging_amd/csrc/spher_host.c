@@ -104,6 +104,21 @@ static void spher_tiling(awefd_model *m)
 {
     const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2];
     m->ntz = (n1 + 63) / 64;
+    const char *ce = getenv("AWEFD_COL");
+    m->col = m->ndim == 3 && !(ce && atoi(ce) == 0);
+    if (m->col) {
+        /* columns of 64 r x AWEFD_SPHER_CRX theta, phi chunks of about
+         * AWEFD_SPHER_CHUNK planes (default 12: 0.563 ms per step on the
+         * 460 x 572 x 572 shell; 6-8 planes 0.577, 16 0.567); blocks: 8 XCD
+         * shares of ceil(ntx / 8) theta tiles */
+        const char *ch = getenv("AWEFD_SPHER_CHUNK");
+        const long chunk = ch && atol(ch) > 0 ? atol(ch) : 12;
+        m->ntx = (n2 + AWEFD_SPHER_CRX - 1) / AWEFD_SPHER_CRX;
+        long nty = (n3 + chunk - 1) / chunk;
+        m->nty = (int)(nty < 1 ? 1 : nty > n3 ? n3 : nty);
+        m->nblk = 8 * m->ntz * ((m->ntx + 7) / 8) * m->nty;
+        return;
+    }
     m->ntx = (n2 + AWEFD_SPHER_TX - 1) / AWEFD_SPHER_TX;
     m->nty = 1;
     if (m->ndim == 3) {
@@ -457,6 +472,7 @@ static void spher_args(const awefd_model *m, awefd_spher_args *a)
     a->ntx = m->ntx;
     a->nty = m->nty;
     a->nblk = m->nblk;
+    a->col = m->col;
     a->nsr = s->nsr;
     a->nst = s->nst;
     a->nsp = s->nsp;

@@ -471,6 +471,164 @@ __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args 
     }
 }
 
+/* 3-D spherical step in columns: one wave per block owns 64 r nodes (one per
+ * lane) of CRX theta rows and streams a chunk of phi planes, with every
+ * per-plane input (previous field, (dt v)^2, the r-halo value of each row,
+ * the two theta-halo rows) loaded CPD planes ahead into rotating register
+ * slots.  Block b runs on XCD b % 8; XCD k owns the theta tiles
+ * [ntx k / 8, ntx (k + 1) / 8), taken r tile fastest, then theta, then phi
+ * chunk, so the theta-halo rows and the chunk prologues of neighbouring
+ * columns are re-read from that XCD's L2.  The arithmetic, its order and the
+ * selections are k_spher_step3's, node for node. */
+#ifndef AWEFD_SPHER_CPD
+#define AWEFD_SPHER_CPD 3
+#endif
+constexpr int CRX = AWEFD_SPHER_CRX;
+constexpr int CPD = AWEFD_SPHER_CPD;
+constexpr int CNS = CPD + 1;                               /* stage slots */
+constexpr int CQR = ((3 + CPD + CPD) / CNS) * CNS;         /* ring slots: phi-1 .. phi+1+CPD */
+
+__device__ __forceinline__ void col_plane(float (&q)[CRX], float (&v)[CRX], float (&zh)[CRX], float (&hal)[2],
+                                          rsrc_t rQ, rsrc_t rV, rsrc_t rP, unsigned vz, unsigned vzh, int x0, int p,
+                                          int pbase, unsigned rowb, unsigned psz)
+{
+#pragma unroll
+    for (int j = 0; j < CRX; j++) {
+        const unsigned s = soff(x0 + j, p, pbase, rowb, psz);
+        q[j] = bld<NT>(rQ, vz, s);
+        v[j] = bld<NT>(rV, vz, s);
+        zh[j] = bld(rP, vzh, s);
+    }
+    hal[0] = bld(rP, vz, soff(x0 - 1, p, pbase, rowb, psz));
+    hal[1] = bld(rP, vz, soff(x0 + CRX, p, pbase, rowb, psz));
+}
+
+__global__ __launch_bounds__(WAVE) void k_spher_col3(const awefd_spher_args a)
+{
+    extern __shared__ double col_dtab[];
+    const int lane = threadIdx.x;
+    const int bid = blockIdx.x;
+    const int xcd = bid & 7, slot = bid >> 3;
+    const int xb = (int)((long)a.ntx * xcd / 8), nxk = (int)((long)a.ntx * (xcd + 1) / 8) - xb;
+    if (nxk == 0) return;
+    const int tzi = slot % a.ntz;
+    const int txi = xb + (slot / a.ntz) % nxk;
+    const int tyi = slot / (a.ntz * nxk);
+    if (tyi >= a.nty) return;
+    const int n1 = a.n1, n2 = a.n2, n3 = a.n3, nb = a.nb;
+    const int z0 = tzi * WAVE;
+    const int ir = z0 + lane;
+    const bool zin = ir < n1;
+    const int x0 = txi * CRX;
+    const int y0 = (int)((long)tyi * n3 / a.nty);
+    const int y1 = (int)((long)(tyi + 1) * n3 / a.nty);
+    const long sy = (long)a.ld * n2;
+    const unsigned rowb = (unsigned)a.ld * 4u;
+    const unsigned psz = (unsigned)(sy * 4);
+    const int pbase = max(y0 - 1, 0);
+    const int ptop = min(y1 + 1, n3);
+    const unsigned nbytes = (unsigned)(ptop - pbase) * psz;
+    const rsrc_t rP = mk_rsrc(a.P + pbase * sy, nbytes);
+    const rsrc_t rQ = mk_rsrc(a.Q + pbase * sy, nbytes);
+    const rsrc_t rS = mk_rsrc(a.S + pbase * sy, nbytes);
+    const rsrc_t rV = mk_rsrc(a.vv + pbase * sy, nbytes);
+
+    const unsigned vz = zin ? (unsigned)ir * 4u : OOB;
+    int zz = -1;
+    if (lane == 0) zz = z0 - 1;
+    else if (lane == WAVE - 1) zz = z0 + WAVE;
+    const unsigned vzh = (zz >= 0 && zz < n1) ? (unsigned)zz * 4u : OOB;
+
+    const bool rint = ir >= NOP && ir < n1 - NOP;
+    const bool rsh = ir < W || ir >= n1 - W;
+    const float m2 = zin ? a.mr2[ir] : 0.f;
+    const float m1 = zin ? a.mr1[ir] : 0.f;
+    double mt1[CRX], mt2[CRX];
+    unsigned tint_m = 0, tsh_m = 0, tin_m = 0;
+#pragma unroll
+    for (int j = 0; j < CRX; j++) {
+        const int ith = x0 + j;
+        const bool ok = zin && ith < n2;
+        mt1[j] = ok ? a.mt1[(long)ith * n1 + ir] : 0.0;
+        mt2[j] = ok ? a.mt2[(long)ith * n1 + ir] : 0.0;
+        tin_m |= ith < n2 ? 1u << j : 0u;
+        tint_m |= (ith >= NOP && ith < n2 - NOP) ? 1u << j : 0u;
+        tsh_m |= (ith < W || ith >= n2 - W) ? 1u << j : 0u;
+    }
+    /* the sponge table (nb + 1 doubles) in LDS */
+    for (int i = lane; i <= nb; i += WAVE) col_dtab[i] = a.damp[i];
+    __syncthreads();
+    const bool sp_rt = z0 < nb || z0 + WAVE - 1 > n1 - nb || x0 < nb || x0 + CRX - 1 > n2 - nb;
+    const float tdr = a.tdr, dr2 = a.dr2, tdt = a.tdt, dt2 = a.dt2, dp2 = a.dp2;
+
+    /* current field: plane p in ring slot (p - y0 + 1) % CQR; stage slot of
+     * plane p: (p - y0) % CNS */
+    float q[CRX][CQR];
+    float sq_[CNS][CRX], sv_[CNS][CRX], sz_[CNS][CRX], sh_[CNS][2];
+#pragma unroll
+    for (int k = 0; k < 2 + CPD; k++)
+#pragma unroll
+        for (int j = 0; j < CRX; j++) q[j][k] = bld(rP, vz, soff(x0 + j, y0 - 1 + k, pbase, rowb, psz));
+#pragma unroll
+    for (int k = 0; k < CPD; k++)
+        col_plane(sq_[k], sv_[k], sz_[k], sh_[k], rQ, rV, rP, vz, vzh, x0, y0 + k, pbase, rowb, psz);
+
+    for (int yg = y0; yg < y1; yg += CQR) {
+#pragma unroll
+        for (int R = 0; R < CQR; R++) {
+            const int y = yg + R;
+            if (y >= y1) break;
+            const int B = R % CNS;
+            const int C0 = (R + 1) % CQR;
+            if (y + CPD < y1) {
+                const int Bn = (R + CPD) % CNS;
+#pragma unroll
+                for (int j = 0; j < CRX; j++)
+                    q[j][(R + 2 + CPD) % CQR] = bld(rP, vz, soff(x0 + j, y + 1 + CPD, pbase, rowb, psz));
+                col_plane(sq_[Bn], sv_[Bn], sz_[Bn], sh_[Bn], rQ, rV, rP, vz, vzh, x0, y + CPD, pbase, rowb, psz);
+            }
+            const bool pint = y >= NOP && y < n3 - NOP;
+            const bool psh = y < W || y >= n3 - W;
+            const bool spw = sp_rt || y < nb || y > n3 - nb;   /* wave-uniform */
+#pragma unroll
+            for (int j = 0; j < CRX; j++) {
+                if (!((tin_m >> j) & 1u)) continue;
+                const float c = q[j][C0];
+                const float rm = shr1(sz_[B][j], c);
+                const float rp = shl1(sz_[B][j], c);
+                const float tm = j > 0 ? q[j - 1][C0] : sh_[B][0];
+                const float tp = j < CRX - 1 ? q[j + 1][C0] : sh_[B][1];
+                const float fm = q[j][(C0 + CQR - 1) % CQR];
+                const float fp = q[j][(C0 + 1) % CQR];
+                const float pra = (rp - rm) / tdr;
+                const float ppra = ((rp - 2.f * c) + rm) / dr2;
+                const float pth = (tp - tm) / tdt;
+                const float ppth = ((tp - 2.f * c) + tm) / dt2;
+                const float ppph = ((fp - 2.f * c) + fm) / dp2;
+                const float s1 = m2 * pra + ppra;
+                double acc = (double)s1 + mt1[j] * (double)pth;
+                acc = acc + (double)(m1 * ppth);
+                acc = acc + mt2[j] * (double)ppph;
+                const bool inter = rint && ((tint_m >> j) & 1u) && pint;
+                const float lap = inter ? (float)acc : 0.f;
+                float pp = sq_[B][j];
+                int di = -1;
+                if (spw) {
+                    di = damp_index<3>(ir, x0 + j, y, n1, n2, n3, nb);
+                    pp = damp(pp, di, col_dtab);
+                }
+                const float f0 = (sv_[B][j] * lap + 2.f * c) - pp;
+                float f = spw ? damp(f0, di, col_dtab) : f0;
+                if (a.fsrf) f = 0.f;   /* freeBC_3D zeroes the whole field */
+                const bool sh = rsh || ((tsh_m >> j) & 1u) || psh;
+                const unsigned s = soff(x0 + j, y, pbase, rowb, psz);
+                bst<NT>(rQ, sh ? OOB : vz, s, f);
+                bst<NT>(rS, sh ? vz : OOB, s, f0);
+            }
+        }
+    }
+}
+
 } // namespace
 
 extern "C" int awefd_launch_spher_inject(const awefd_spher_args *a, void *stream)
@@ -483,7 +641,9 @@ extern "C" int awefd_launch_spher_inject(const awefd_spher_args *a, void *stream
 extern "C" int awefd_launch_spher_step(const awefd_spher_args *a, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
-    if (a->ndim == 3) {
+    if (a->ndim == 3 && a->col) {
+        hipLaunchKernelGGL(k_spher_col3, dim3(a->nblk), dim3(WAVE), sizeof(double) * (size_t)(a->nb + 1), s, *a);
+    } else if (a->ndim == 3) {
         hipLaunchKernelGGL(k_spher_step3, dim3(a->nblk), dim3(NTHR), 0, s, *a);
     } else {
         dim3 g((a->n1 + 63) / 64, (a->n2 + 3) / 4);
