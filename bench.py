@@ -64,26 +64,39 @@ WORKLOADS = {
                    traffic="cart3d_step_traffic.json",
                    desc="cart3d 256x512x512 (z,x,y) h=5m nb=40 nt=%d dabc=y fdorder=4, "
                         "padded 336x592x592, 1 source, 128x128 receivers"),
-    "spher3d": dict(kernel="k_spher_step3",
+    "spher3d": dict(kernel="k_spher_col3",
                     traffic="spher3d_step_traffic.json",
                     desc="spher3d 400x512x512 (r,theta,phi) r0=3387.5km dr=5m nb=30 nt=%d "
                          "expl=n, padded 460x572x572, 1 source, 256x256 receivers"),
 }
 
 
+def _traffic_file(name):
+    if not name:
+        return {}
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return {}
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
 def _traffic(name):
     """Calibrated HBM bytes per launch of the step kernel from the committed
     counter summary."""
-    if not name:
-        return None
-    path = os.path.join(ROOT, "profiles", name)
-    if not os.path.exists(path):
-        return None
-    try:
-        with open(path) as f:
-            return json.load(f).get("hbm_bytes_per_launch")
-    except (OSError, ValueError):
-        return None
+    return _traffic_file(name).get("hbm_bytes_per_launch")
+
+
+def _traffic_scaled(name, algorithmic):
+    """Calibrated HBM bytes of `algorithmic` bytes of work, from the
+    traffic/algorithmic ratio a committed counter summary measured for the
+    same kernel on a smaller sample (the batch kernels' traffic per cell does
+    not depend on the number of shots)."""
+    ratio = _traffic_file(name).get("traffic_over_algorithmic")
+    return None if ratio is None else int(round(ratio * algorithmic))
 
 
 def _oracle():
@@ -264,7 +277,9 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True, fixed=No
                s_per_batch=round(secs, 4), ms_per_time_step=round(secs / nt * 1e3, 5), tile_rx=rx, group_mb=grp,
                steps_per_launch=depth,
                roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kern,
+                         "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": _traffic_scaled("batch2d_pass_traffic.json", 20 * cells) if depth == 3 else None,
+                         "traffic_per": "one pass of three steps over all shots", "kernel": kern,
                          "bytes_per_run": bytes_run,
                          "timing": "the timed graph: %d passes of %d steps at 20 B/node and %d single steps at "
                                    "16 B/node over the run's wall time (fields of one shot fit the 256 MB "
@@ -394,7 +409,9 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cp
                    batched_gcells_per_s_per_propagation=round(3 * cells * nt * nshot / (t1 - t0) / 1e9, 3),
                    speedup_vs_per_shot_loop=round(bat / loop, 2),
                    roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                             "frac": round(achieved / HBM_PEAK_GBS, 4),
+                             "traffic": _traffic_scaled("adjoint2d_traffic.json", bytes_grad * nshot),
+                             "traffic_per": "one batched gradient of all shots",
                              "kernel": "k_adj2d_batch<2> with k_cart_batch2 (forward, recompute)",
                              "bytes_per_gradient": bytes_grad,
                              "timing": "the timed batched gradient (forward graph, residuals, backward graph): "
