@@ -90,3 +90,36 @@ def test_sinc_weights_equal_the_oracle(orc):
             lib.awefd_sinc_axis(C.c_float(q), zaxis, a.ctypes.data_as(f), C.byref(fa), C.byref(na))
             orc.lib().orc_sinc_axis(C.c_float(q), zaxis, b.ctypes.data_as(f), C.byref(fb), C.byref(nb))
             assert a.tobytes() == b.tobytes() and (fa.value, na.value) == (fb.value, nb.value), (q, zaxis)
+
+
+class _Udiv(C.Structure):
+    _fields_ = [("m", C.c_uint), ("l", C.c_int)]
+
+
+def test_magic_division_is_exact():
+    """The spherical shell kernel decodes node numbers with the host's
+    multiply-high divisors: (umulhi(n, m) + n) >> l must equal n // d for
+    every n < 2^31.  Checked for d = 1..3000, the padded axis sizes of the
+    measured grids and random d up to 2^31 - 1, at n = 0, the multiples of d
+    and their neighbours near 0 and 2^31, and random n."""
+    import numpy as np
+
+    lib = L.load()
+    f = lib.awefd_magic_div
+    f.restype = _Udiv
+    f.argtypes = [C.c_uint]
+    rng = np.random.default_rng(5)
+    ds = list(range(1, 3001)) + [448, 460, 560, 572, 12, 1 << 20, (1 << 31) - 1] \
+        + [int(x) for x in rng.integers(1, (1 << 31) - 1, 200)]
+    top = (1 << 31) - 1
+    for d in ds:
+        r = f(d)
+        m, l = np.uint64(r.m), np.uint64(r.l)
+        k = np.arange(0, 40, dtype=np.uint64)
+        n = np.concatenate([k, k * np.uint64(d), k * np.uint64(d) + np.uint64(1),
+                            np.maximum(k * np.uint64(d), np.uint64(1)) - np.uint64(1),
+                            np.uint64(top) - k, (np.uint64(top) // np.uint64(d) - k) * np.uint64(d),
+                            rng.integers(0, top, 64, dtype=np.int64).astype(np.uint64)])
+        n = n[n <= np.uint64(top)]
+        q = (((n * m) >> np.uint64(32)) + n) >> l
+        assert np.array_equal(q, n // np.uint64(d)), d

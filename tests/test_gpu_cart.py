@@ -132,3 +132,36 @@ def test_cart2d_graph_replays_setters_and_eager_path(orc):
     m.run_dev(buf.data_ptr())
     np.testing.assert_array_equal(buf.cpu().numpy(), ref2)
     m.close()
+
+
+@pytest.mark.parametrize("nz,kw", [
+    (60, dict()),                                   # padded z 72 = 64 + 8
+    (68, dict(fdorder=8)),                          # 80 = 64 + 16, nop 4
+    (60, dict(free=True, sse=False)),
+    (60, dict(dabc=False)),
+    (60, dict(fdorder=12)),                         # nop 6: 72 = 64 + 8, 2 nop halo lanes of 16
+    (53, dict()),                                   # 65 = 64 + 1: z-high ABC in its own pass
+    (4, dict()),                                    # 16: the whole z axis in 16-lane rows
+    (60, dict(cden=False)),                         # variable density
+])
+def test_cart3d_last_z_tile_in_rows(orc, nz, kw, monkeypatch, capfd):
+    """Grids whose last z tile holds at most 16 nodes run it as four 16-lane
+    rows of 4 x tiles per wave (z neighbours by row shifts, each row's
+    addresses offset by its x tile).  82 x rows: 41 x tiles, so some XCDs
+    own a group of fewer than 4; sources and receivers in the last z tile."""
+    monkeypatch.setenv("AWEFD_VERBOSE", "1")
+    nx, ny, nb = 70, 18, 6
+    den = kw.pop("cden", True)
+    par, vel, sou, rec, wav = _case3d(nz=nz, nx=nx, ny=ny, nt=50, nb=nb, cden=den, **kw)
+    zdeep = (nz - 1.6) * 5.0
+    sou = np.array([[nx * 2.5 + 1.3, ny * 2.5 - 2.1, min(6 * 5.0 + 0.7, zdeep)],
+                    [3 * 5.0 + 0.4, 4 * 5.0 + 2.2, zdeep], [(nx - 2) * 5.0, (ny - 3) * 5.0, zdeep]], np.float32)
+    ix, iy = np.meshgrid(np.arange(0, nx, 4), np.arange(0, ny, 4), indexing="ij")
+    rec = np.concatenate([
+        np.stack([ix.ravel() * 5.0, iy.ravel() * 5.0, np.full(ix.size, min(4 * 5.0, zdeep))], 1),
+        np.stack([ix.ravel() * 5.0 + 1.1, iy.ravel() * 5.0 + 0.6, np.full(ix.size, (nz - 1.3) * 5.0)], 1),
+    ]).astype(np.float32)
+    w3 = np.stack([wav, -0.5 * wav, 2 * wav], 1).astype(np.float32)   # one wavelet per source
+    _check("cart3d", par, vel, sou, rec, w3, orc, den=None if den else _density(vel, 3))
+    err = capfd.readouterr().err
+    assert "last z tile in 16-lane rows" in err, err

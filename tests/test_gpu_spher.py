@@ -119,3 +119,34 @@ def test_spher3d_many_sources_same_node(orc):
     # region, whose reflectivity is 0
     src = [[23.5, 13.0, 12.0], [23.5, 13.0, 12.0], [23.25, 13.5, 12.25], [6.0, 6.0, 20.0]]
     _check("spher3d", par, vel, _coords(o, d, src), rec, wav, orc)
+
+
+@pytest.mark.parametrize("which", ["interior", "shell_receivers", "receiver_on_source", "receivers_on_sources"])
+def test_spher3d_receivers_in_shell_and_on_sources(orc, which):
+    """Receivers whose corners lie in the shell (r, theta and phi faces: their
+    values come from the shell pass) or on injection nodes (read after the
+    step, before the next injection), with nb = 4 so that the shell reaches
+    into the model."""
+    n, nb = [30, 28, 26], 4
+    par, vel, sou, rec, wav = _case(3, n=n, nb=nb, nt=60)
+    o, d = par["o"], par["d"]
+    src = [[n[0] - 6.5, n[1] / 2 + 0.4, n[2] / 2 - 0.7]]
+    pts = [[n[0] - 8.6, t, 13.3] for t in np.linspace(3.0, n[1] - 4.0, 5)]   # corners off the shell
+    if which == "shell_receivers":
+        # padded r 4.5 and 32.5 of 38, theta 4.25 of 36, phi 28.5 of 34
+        pts += [[0.5, 10.0, 10.0], [n[0] - 1.5, 12.0, 9.0], [10.0, 0.25, 9.0], [11.0, 13.0, n[2] - 1.5]]
+    if which == "receiver_on_source":
+        pts += [[n[0] - 6.5, n[1] / 2 + 0.4, n[2] / 2 - 0.7]]
+    if which == "receivers_on_sources":
+        # two sources; three receivers share corners with them (two on the
+        # sources, one elsewhere in the first source's cell)
+        src += [[10.2, 7.3, 9.9]]
+        pts += [[n[0] - 6.5, n[1] / 2 + 0.4, n[2] / 2 - 0.7], [10.2, 7.3, 9.9],
+                [n[0] - 6.5, n[1] / 2 + 0.9, n[2] / 2 - 0.7]]
+    par = dict(par, expl=True)
+    _check("spher3d", par, vel, _coords(o, d, src), _coords(o, d, pts), wav, orc)
+
+
+def test_spher3d_source_in_shell(orc):
+    par, vel, sou, rec, wav = _case(3, nb=4, nt=50, src=[[0.5, 14.0, 13.0]])
+    _check("spher3d", par, vel, sou, rec, wav, orc)

@@ -164,12 +164,20 @@ static void cart_tiling(awefd_model *m)
         m->nty = 1;
     }
     m->nblk = m->ntz * m->ntx * m->nty;
-    /* the column kernel's blocks: 8 XCD shares of ceil(ntx / 8) x tiles */
-    if (m->col) m->nblk = 8 * m->ntz * ((m->ntx + 7) / 8) * m->nty;
+    /* the column kernel's blocks: 8 XCD shares of ceil(ntx / 8) x tiles.
+     * When the last z tile holds at most 16 nodes (n1 % 64 <= 16: 336 =
+     * 5 x 64 + 16), a wave would run 48 or more idle lanes there; instead
+     * one wave takes that tile for 4 x tiles at once, 16 lanes per x tile
+     * (AWEFD_COL_TAIL=0: one x tile per wave), and an XCD's share is laid out
+     * in groups of 4 x tiles: their 4 (ntz - 1) full tiles, then the tail. */
+    const char *te = getenv("AWEFD_COL_TAIL");
+    m->ztail = m->col && n1 % 64 != 0 && n1 % 64 <= 16 && !(te && atoi(te) == 0);
+    if (m->col && !m->ztail) m->nblk = 8 * m->ntz * ((m->ntx + 7) / 8) * m->nty;
+    if (m->ztail) m->nblk = 8 * (4 * (m->ntz - 1) + 1) * (((m->ntx + 7) / 8 + 3) / 4) * m->nty;
     if (getenv("AWEFD_VERBOSE"))
         fprintf(stderr, "awefd: step kernel%s tiles %d(z) x %d(x) x %d(y, <= %d planes) = %d blocks, "
-                "target %d\n", m->col ? " (columns)" : "", m->ntz, m->ntx, m->nty, m->ychunk, m->nblk,
-                m->target_blocks);
+                "target %d\n", m->col ? (m->ztail ? " (columns, last z tile in 16-lane rows)" : " (columns)") : "",
+                m->ntz, m->ntx, m->nty, m->ychunk, m->nblk, m->target_blocks);
 }
 
 int awefd_create(const awefd_params *p, awefd_model **out)
@@ -812,6 +820,7 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
     a->ychunk = m->ychunk;
     a->nty = m->nty;
     a->col = m->col;
+    a->ztail = m->ztail;
     a->nblk = m->nblk;
     a->dabc = m->sponge;
     a->nb = m->nb;

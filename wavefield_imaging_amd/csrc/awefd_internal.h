@@ -75,6 +75,8 @@ typedef struct {
     /* tiling */
     int tz, ntz, ntx, ychunk, nty, nblk; /* ychunk: longest y chunk      */
     int col;                 /* 3-D: the column kernel (one wave per block)    */
+    int ztail;               /* column kernel: the last z tile (n1 % 64 <= 16
+                                nodes) as 16-lane rows, 4 x tiles a wave */
     /* sponge: per-axis factors, 1 outside the strips */
     int dabc, nb;
     const float *sp1, *sp2, *sp3;
@@ -167,6 +169,13 @@ typedef struct {
 } awefd_copy_job;
 int awefd_launch_copy_jobs(const awefd_copy_job *jobs, const long *cbeg, int njob, long nchunk, void *stream);
 
+/* n / d = (umulhi(n, m) + n) >> l for every n < 2^31 (awefd_magic_div). */
+typedef struct {
+    unsigned m;
+    int l;
+} awefd_udiv;
+awefd_udiv awefd_magic_div(unsigned d);
+
 /* Polar (2-D) and spherical (3-D) kernels.  Axes: 1 = r (fastest), 2 = theta,
  * 3 = phi.  Nodes within AWEFD_SPHER_SHELL layers of a face form the shell:
  * the step kernel stores their values before the one-way boundary (S) and the
@@ -209,8 +218,12 @@ typedef struct {
     const float *bthl, *bthh, *brl, *brh, *bphl, *bphh;
     /* step tiling; col: the 3-D column kernel (one wave per block) */
     int ntz, ntx, nty, nblk, col;
-    /* shell enumeration: counts of the r-face, theta-face and phi-face groups */
-    long nsr, nst, nsp;
+    /* shell enumeration: counts of the r-face, theta-face and phi-face
+     * groups (below 2^31 in all), and the divisions that decode a node
+     * number: by the r-face layer count, n2, the inner r count, the
+     * theta-face layer count and the inner theta count */
+    unsigned nsr, nst, nsp;
+    awefd_udiv dv_rc, dv_n2, dv_rin, dv_tc, dv_tin;
     /* injection: nodes (flat offsets) with their contributions in order */
     int ninode;
     const long *inode;

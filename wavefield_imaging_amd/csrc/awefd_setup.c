@@ -104,6 +104,21 @@ void awefd_sponge_axis(const float *w, int nb, int n, float *f)
     }
 }
 
+/* Division by d (1 <= d < 2^31) as a multiply-high, an add and a shift,
+ * exact for every n < 2^31 (Granlund and Montgomery, "Division by invariant
+ * integers using multiplication", PLDI 1994, the round-up multiplier):
+ * l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+ * n / d = (umulhi(n, m) + n) >> l; the sum stays below 2^32. */
+awefd_udiv awefd_magic_div(unsigned d)
+{
+    awefd_udiv r;
+    int l = 0;
+    while (l < 31 && (1u << l) < d) l++;
+    r.l = l;
+    r.m = (unsigned)(((1ULL << 32) * ((1ULL << l) - d)) / d + 1);
+    return r;
+}
+
 /* Spherical sponge factor, spher_utils.c:665-667 (double). */
 double awefd_spher_damp(int i, int nb)
 {

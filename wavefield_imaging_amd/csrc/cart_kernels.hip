@@ -495,7 +495,9 @@ __device__ __forceinline__ void cart_step_body(const awefd_cart_args &a, float *
 
 /* ---- The 3-D step in columns ----
  *
- * A block is one wave: 64 z nodes (one per lane) of RX x rows, streaming a
+ * A block is one wave: 64 z nodes (one per lane) of RX x rows (or, for the
+ * last z tile when it holds at most 16 nodes, 16 z nodes of 4 x tiles of RX
+ * rows: four 16-lane segments, z neighbours by row shifts), streaming a
  * y range of the grid (the whole axis unless the grid is too small to fill
  * the chip with columns).  Every per-plane input (previous field, (v dt)^2,
  * the current field's head plane, its z- and x-halo values, density
@@ -519,11 +521,11 @@ constexpr int col_ring()
 
 /* loads of plane y into stage slot S: previous field, (v dt)^2, the z-halo
  * value of each row and the 2 NOP x-halo rows of the current field */
-template <int NOP, int RX, bool VD>
+template <int NOP, int RX, bool VD, int SEG, int NH>
 __device__ __forceinline__ void col_stage(float (&p)[RX], float (&v)[RX], float (&zh)[RX], float (&hal)[2 * NOP],
                                           float4 (&g)[RX], rsrc_t rP, rsrc_t rV, rsrc_t rC, rsrc_t rR, unsigned vz,
-                                          unsigned vzh, unsigned vz4, int x0, int y, int ybase, unsigned rowb,
-                                          unsigned psz)
+                                          unsigned vzh, unsigned vz4, const unsigned (&vhx)[NH], int x0, int y,
+                                          int ybase, unsigned rowb, unsigned psz)
 {
 #pragma unroll
     for (int j = 0; j < RX; j++) {
@@ -535,40 +537,54 @@ __device__ __forceinline__ void col_stage(float (&p)[RX], float (&v)[RX], float 
     }
 #pragma unroll
     for (int h = 0; h < 2 * NOP; h++) {
-        const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
-        hal[h] = bld(rC, vz, soff(hx, y, ybase, rowb, psz));
+        if (SEG == WAVE) {
+            const int hx = h < NOP ? x0 - NOP + h : x0 + RX + (h - NOP);
+            hal[h] = bld(rC, vz, soff(hx, y, ybase, rowb, psz));
+        } else {
+            hal[h] = bld(rC, vhx[h], (unsigned)(y - ybase) * psz);
+        }
     }
 }
 
-template <int NOP, bool SSE, bool VD, int RX, int PD>
-__device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *Pout, const float *Cin, int it,
-                                              float *trow, int bid, float *lds)
+/* z shifts by one node within the wave's z segments: the whole wave
+ * (SEG 64) or each 16-lane row (SEG 16); the lane without a source keeps
+ * `old` */
+template <int SEG>
+__device__ __forceinline__ float zshr(float old, float v)
+{
+    if (SEG == WAVE) return shr1(old, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), 0x111, 0xf, 0xf, false));
+}
+template <int SEG>
+__device__ __forceinline__ float zshl(float old, float v)
+{
+    if (SEG == WAVE) return shl1(old, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), 0x101, 0xf, 0xf, false));
+}
+
+/* One column chunk: z tile tzi, x tile txi (RX rows), y chunk tyi.  SEG 64:
+ * one z segment of 64 lanes.  SEG 16: the last z tile of a grid with
+ * n1 % 64 <= 16, four 16-lane z segments, segment s taking x tile txi + s
+ * (s < nsub); every address then carries the segment's row offset. */
+template <int NOP, bool SSE, bool VD, int RX, int PD, int SEG>
+__device__ __forceinline__ void col_tile(const awefd_cart_args &a, float *Pout, const float *Cin, int it, int tzi,
+                                         int txi, int tyi, int nsub, float *lds)
 {
     constexpr int QR = col_ring<NOP, PD>();
     constexpr int NS = PD + 1;
+    constexpr int NSEG = WAVE / SEG;
+    constexpr int RXT = RX * NSEG;                     /* x rows of the wave */
     const int lane = threadIdx.x;
-    if (bid >= a.nblk) {
-        cart_extract<3>(a, Cin, trow, (bid - a.nblk) * WAVE + lane);
-        return;
-    }
-    /* Block b runs on XCD b % 8.  XCD k owns the columns of x tiles
-     * [ntx k / 8, ntx (k + 1) / 8) and takes them z tile fastest, then x,
-     * then y chunk: the columns that share x-halo rows, and the chunks that
-     * share prologue planes, run on one XCD close together in time, so those
-     * re-reads hit its L2, and the eight XCDs stream the same planes at a
-     * time.  Blocks past an XCD's share (when 8 does not divide ntx) exit. */
-    const int xcd = bid & 7, slot = bid >> 3;
-    const int xb = (int)((long)a.ntx * xcd / 8), nxk = (int)((long)a.ntx * (xcd + 1) / 8) - xb;
-    if (nxk == 0) return;
-    const int tzi = slot % a.ntz;
-    const int txi = xb + (slot / a.ntz) % nxk;
-    const int tyi = slot / (a.ntz * nxk);
-    if (tyi >= a.nty) return;
+    const int zoff = lane % SEG, sub = lane / SEG;
+    const bool subok = sub < nsub;
     const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
     const int z0 = tzi * WAVE;
-    const int iz = z0 + lane;
-    const bool zin = iz < n1;
-    const int x0 = txi * RX;
+    const int iz = z0 + zoff;
+    const bool zin = iz < n1 && subok;
+    const int x0 = txi * RX;                           /* first row of the wave */
+    const int xl = x0 + sub * RX;                      /* first row of this lane's segment */
     const int y0 = (int)((long)tyi * n3 / a.nty);
     const int y1 = (int)((long)(tyi + 1) * n3 / a.nty);
     const long sy = (long)a.ld * n2;
@@ -582,34 +598,49 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
     const rsrc_t rV = mk_rsrc(a.vel + ybase * sy, nbytes);
     const rsrc_t rR = mk_rsrc(VD ? (const void *)(a.rho + 4 * ybase * sy) : (const void *)a.vel,
                               VD ? 4u * nbytes : 0u);
-    const unsigned vz = zin ? (unsigned)iz * 4u : OOB;
-    const unsigned vz4 = zin ? (unsigned)iz * 16u : OOB;
+    const unsigned xo = (unsigned)(sub * RX) * rowb;  /* the segment's row offset */
+    const unsigned vz = zin ? (unsigned)iz * 4u + xo : OOB;
+    const unsigned vz4 = zin ? (unsigned)iz * 16u + 4u * xo : OOB;
     int zz = -1;
-    if (lane < NOP) zz = z0 - 1 - lane;
-    else if (lane >= WAVE - NOP) zz = z0 + WAVE + (WAVE - 1 - lane);
-    const unsigned vzh = (zz >= 0 && zz < n1) ? (unsigned)zz * 4u : OOB;
+    if (zoff < NOP) zz = z0 - 1 - zoff;
+    else if (zoff >= SEG - NOP) zz = z0 + SEG + (SEG - 1 - zoff);
+    const unsigned vzh = (subok && zz >= 0 && zz < n1) ? (unsigned)zz * 4u + xo : OOB;
+    /* SEG 16: the x-halo rows by lane (a row below the grid is dropped) */
+    unsigned vhx[SEG == WAVE ? 1 : 2 * NOP] = {OOB};
+    if (SEG != WAVE) {
+#pragma unroll
+        for (int h = 0; h < 2 * NOP; h++) {
+            const int hx = h < NOP ? xl - NOP + h : xl + RX + (h - NOP);
+            vhx[h] = (zin && hx >= 0) ? (unsigned)hx * rowb + (unsigned)iz * 4u : OOB;
+        }
+    }
 
     const bool zint = iz >= NOP && iz < n1 - NOP;
     const bool zsp = a.dabc && zin && (iz < a.nb || iz >= n1 - a.nb);
     const float spz = zsp ? a.sp1[iz] : 1.f;
-    const bool zsp_w = a.dabc && (z0 < a.nb || z0 + WAVE > n1 - a.nb);
+    const bool zsp_w = a.dabc && (z0 < a.nb || z0 + SEG > n1 - a.nb);
     const bool zdef = a.zsep && iz >= n1 - NOP - 2;
     float spx[RX];
     unsigned xint_m = 0, xdef_m = 0, xin_m = 0;
     bool xsp_w = false;
 #pragma unroll
     for (int j = 0; j < RX; j++) {
-        const int ix = x0 + j;
-        const bool xsp = a.dabc && ix < n2 && (ix < a.nb || ix >= n2 - a.nb);
-        xsp_w = xsp_w || xsp;
+        const int ix = xl + j;
+        const bool xsp = a.dabc && subok && ix < n2 && (ix < a.nb || ix >= n2 - a.nb);
         spx[j] = xsp ? a.sp2[ix] : 1.f;
-        xin_m |= ix < n2 ? 1u << j : 0u;
+        xin_m |= (subok && ix < n2) ? 1u << j : 0u;
         xint_m |= (ix >= NOP && ix < n2 - NOP) ? 1u << j : 0u;
         xdef_m |= (a.abc && (ix <= NOP + 1 || ix >= n2 - NOP - 2)) ? 1u << j : 0u;
     }
+    /* wave-uniform: any row of any segment in an x sponge strip */
+#pragma unroll
+    for (int k = 0; k < RXT; k++) {
+        const int ix = x0 + k;
+        xsp_w = xsp_w || (a.dabc && k < nsub * RX && ix < n2 && (ix < a.nb || ix >= n2 - a.nb));
+    }
     const bool sp_xz = zsp_w || xsp_w;
-    const bool zlow = a.abc && tzi == 0 && !a.fsrf;
-    const bool zhigh = a.abc && !a.zsep && tzi == a.ntz - 1;
+    const bool zlow = a.abc && z0 == 0 && !a.fsrf;
+    const bool zhigh = a.abc && !a.zsep && z0 + SEG >= n1;
 
     /* LDS: the y sponge factor of each plane of the range, then (z-edge
      * columns) the z-low and z-high one-way coefficients of each row */
@@ -618,18 +649,22 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
     float *zb_l = lds + ny;
     for (int i = lane; i < ny; i += WAVE) ysp_l[i] = a.dabc ? a.sp3[y0 + i] : 1.f;
     if (zlow || zhigh)
-        for (int i = lane; i < ny * RX; i += WAVE) {
-            const long o = (long)(y0 + i / RX) * n2 + min(x0 + i % RX, n2 - 1);
+        for (int i = lane; i < ny * RXT; i += WAVE) {
+            const long o = (long)(y0 + i / RXT) * n2 + min(x0 + i % RXT, n2 - 1);
             zb_l[2 * i] = zlow ? a.bzl[o] : 0.f;
             zb_l[2 * i + 1] = zhigh ? a.bzh[o] : 0.f;
         }
     __syncthreads();
 
-    /* injection entries of this column, from plane y0 on */
-    const int col = tzi + a.ntz * txi;
-    int ib = a.ibeg[col];
-    const int ie = a.ibeg[col + 1];
-    while (ib < ie && a.inj[ib].y < y0) ib++;
+    /* injection entries of each segment's column, from plane y0 on */
+    int ib[NSEG], ie[NSEG];
+#pragma unroll
+    for (int g = 0; g < NSEG; g++) {
+        const int col = tzi + a.ntz * (txi + g);
+        ib[g] = g < nsub ? a.ibeg[col] : 0;
+        ie[g] = g < nsub ? a.ibeg[col + 1] : 0;
+        while (ib[g] < ie[g] && a.inj[ib[g]].y < y0) ib[g]++;
+    }
 
     /* current field ring (plane p in slot (p - y0 + NOP) % QR) and stage
      * slots (plane y in slot (y - y0) % NS) */
@@ -642,8 +677,8 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
         for (int j = 0; j < RX; j++) q[j][k] = bld(rC, vz, soff(x0 + j, y0 - NOP + k, ybase, rowb, psz));
 #pragma unroll
     for (int k = 0; k < PD; k++)
-        col_stage<NOP, RX, VD>(sp_[k], sv_[k], sz_[k], sh_[k], sg_[k], rP, rV, rC, rR, vz, vzh, vz4, x0, y0 + k,
-                               ybase, rowb, psz);
+        col_stage<NOP, RX, VD, SEG, (SEG == WAVE ? 1 : 2 * NOP)>(sp_[k], sv_[k], sz_[k], sh_[k], sg_[k], rP, rV, rC, rR, vz, vzh, vz4, vhx,
+                                    x0, y0 + k, ybase, rowb, psz);
 
     for (int yg = y0; yg < y1; yg += QR) {
 #pragma unroll
@@ -657,8 +692,8 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
 #pragma unroll
                 for (int j = 0; j < RX; j++)
                     q[j][(R + 2 * NOP + PD) % QR] = bld(rC, vz, soff(x0 + j, y + NOP + PD, ybase, rowb, psz));
-                col_stage<NOP, RX, VD>(sp_[Bn], sv_[Bn], sz_[Bn], sh_[Bn], sg_[Bn], rP, rV, rC, rR, vz, vzh, vz4, x0,
-                                       y + PD, ybase, rowb, psz);
+                col_stage<NOP, RX, VD, SEG, (SEG == WAVE ? 1 : 2 * NOP)>(sp_[Bn], sv_[Bn], sz_[Bn], sh_[Bn], sg_[Bn], rP, rV, rC, rR, vz, vzh,
+                                            vz4, vhx, x0, y + PD, ybase, rowb, psz);
             }
             float cen[RX];
 #pragma unroll
@@ -673,14 +708,14 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
             for (int j = 0; j < RX; j++) {
                 float zm[NOP], zp[NOP];
                 float hlo = sz_[B][j], hhi = sz_[B][j];
-                zm[0] = shr1(hlo, cen[j]);
-                zp[0] = shl1(hhi, cen[j]);
+                zm[0] = zshr<SEG>(hlo, cen[j]);
+                zp[0] = zshl<SEG>(hhi, cen[j]);
 #pragma unroll
                 for (int k = 1; k < NOP; k++) {
-                    hlo = shl1(hlo, hlo);
-                    hhi = shr1(hhi, hhi);
-                    zm[k] = shr1(hlo, zm[k - 1]);
-                    zp[k] = shl1(hhi, zp[k - 1]);
+                    hlo = zshl<SEG>(hlo, hlo);
+                    hhi = zshr<SEG>(hhi, hhi);
+                    zm[k] = zshr<SEG>(hlo, zm[k - 1]);
+                    zp[k] = zshl<SEG>(hhi, zp[k - 1]);
                 }
                 zp1[j] = zp[0];
                 zm1[j] = zm[0];
@@ -720,38 +755,41 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
                 nv[j] = (zint && ((xint_m >> j) & 1u) && yint) ? v : p;
             }
 
-            int e = ib;
-            for (; e < ie && a.inj[e].y == y; e++) {
-                const awefd_inj en = a.inj[e];
-                const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
 #pragma unroll
-                for (int j = 0; j < RX; j++) {
-                    const float inj = nv[j] + amp * en.w * en.w2 * en.w3 * sv_[B][j];
-                    nv[j] = (zin && en.z == iz && en.x == x0 + j) ? inj : nv[j];
+            for (int g = 0; g < NSEG; g++) {
+                int e = ib[g];
+                for (; e < ie[g] && a.inj[e].y == y; e++) {
+                    const awefd_inj en = a.inj[e];
+                    const float amp = a.wav[(long)it * a.wstride + (a.wexpl ? 0 : en.src)];
+#pragma unroll
+                    for (int j = 0; j < RX; j++) {
+                        const float inj = nv[j] + amp * en.w * en.w2 * en.w3 * sv_[B][j];
+                        nv[j] = (zin && en.z == iz && en.x == xl + j) ? inj : nv[j];
+                    }
                 }
+                ib[g] = e;
             }
-            ib = e;
 
             if (zlow || zhigh) {
 #pragma unroll
                 for (int j = 0; j < RX; j++) {
                     if (!((xin_m >> j) & 1u)) continue;
                     if (zlow) {
-                        const float b = zb_l[2 * ((y - y0) * RX + j)];
+                        const float b = zb_l[2 * ((y - y0) * RXT + sub * RX + j)];
 #pragma unroll
                         for (int k = 0; k < NOP; k++) {
                             const int zt = NOP - k;
-                            const float up = shl1(nv[j], nv[j]);
+                            const float up = zshl<SEG>(nv[j], nv[j]);
                             const float w = zp1[j] + (cen[j] - up) * b;
                             nv[j] = iz == zt ? w : nv[j];
                         }
                     }
                     if (zhigh) {
-                        const float b = zb_l[2 * ((y - y0) * RX + j) + 1];
+                        const float b = zb_l[2 * ((y - y0) * RXT + sub * RX + j) + 1];
 #pragma unroll
                         for (int k = 0; k < NOP; k++) {
                             const int zt = n1 - NOP + k - 1;
-                            const float dn = shr1(nv[j], nv[j]);
+                            const float dn = zshr<SEG>(nv[j], nv[j]);
                             const float w = zm1[j] + (cen[j] - dn) * b;
                             nv[j] = iz == zt ? w : nv[j];
                         }
@@ -772,6 +810,49 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
                 bst<NT>(rP, vz, soff(x0 + j, y, ybase, rowb, psz), v);
             }
         }
+    }
+}
+
+template <int NOP, bool SSE, bool VD, int RX, int PD>
+__device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *Pout, const float *Cin, int it,
+                                              float *trow, int bid, float *lds)
+{
+    const int lane = threadIdx.x;
+    if (bid >= a.nblk) {
+        cart_extract<3>(a, Cin, trow, (bid - a.nblk) * WAVE + lane);
+        return;
+    }
+    /* Block b runs on XCD b % 8.  XCD k owns the columns of x tiles
+     * [ntx k / 8, ntx (k + 1) / 8) and takes them z tile fastest, then x,
+     * then y chunk: the columns that share x-halo rows, and the chunks that
+     * share prologue planes, run on one XCD close together in time, so those
+     * re-reads hit its L2, and the eight XCDs stream the same planes at a
+     * time.  Blocks past an XCD's share (when 8 does not divide ntx) exit. */
+    const int xcd = bid & 7, slot = bid >> 3;
+    const int xb = (int)((long)a.ntx * xcd / 8), nxk = (int)((long)a.ntx * (xcd + 1) / 8) - xb;
+    if (nxk == 0) return;
+    if (!a.ztail) {
+        const int tzi = slot % a.ntz;
+        const int txi = xb + (slot / a.ntz) % nxk;
+        const int tyi = slot / (a.ntz * nxk);
+        if (tyi >= a.nty) return;
+        col_tile<NOP, SSE, VD, RX, PD, WAVE>(a, Pout, Cin, it, tzi, txi, tyi, 1, lds);
+        return;
+    }
+    /* groups of 4 x tiles: their 4 (ntz - 1) full z tiles, then one wave for
+     * the last z tile of all 4 (in the same launch: as a kernel of its own,
+     * launched after the full tiles, it cost more than it saved) */
+    const int ntzm = a.ntz - 1, gsz = 4 * ntzm + 1, per = ((nxk + 3) / 4) * gsz;
+    const int tyi = slot / per;
+    if (tyi >= a.nty) return;
+    const int i = slot % per, g = i / gsz, r = i % gsz;
+    if (r < 4 * ntzm) {
+        const int txi = xb + 4 * g + r / ntzm;
+        if (txi >= xb + nxk) return;
+        col_tile<NOP, SSE, VD, RX, PD, WAVE>(a, Pout, Cin, it, r % ntzm, txi, tyi, 1, lds);
+    } else {
+        const int txi = xb + 4 * g;
+        col_tile<NOP, SSE, VD, RX, PD, 16>(a, Pout, Cin, it, ntzm, txi, tyi, min(4, xb + nxk - txi), lds);
     }
 }
 
@@ -1429,8 +1510,9 @@ int launch_step(const awefd_cart_args *a, bool vd, hipStream_t s)
 {
     if (NDIM == 3 && a->col) {
         /* LDS: the y sponge factor and two z-face coefficients per row of
-         * each plane of the longest chunk */
-        const size_t lds = sizeof(float) * (size_t)a->ychunk * (1 + 2 * AWEFD_COL_RX);
+         * each plane of the longest chunk (4 x tiles of rows for the waves
+         * that take the last z tile of 4 x tiles) */
+        const size_t lds = sizeof(float) * (size_t)a->ychunk * (1 + 2 * AWEFD_COL_RX * (a->ztail ? 4 : 1));
         const int nx = (a->nr + WAVE - 1) / WAVE;
         if (vd)
             hipLaunchKernelGGL((k_cart_col<NOP, SSE, true>), dim3(a->nblk + nx), dim3(WAVE), lds, s, *a);

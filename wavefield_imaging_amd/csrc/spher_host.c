@@ -42,7 +42,7 @@ typedef struct {
     int inj_dirty;
     int *d_rrtp;                     /* receiver map in (r, theta, phi) order     */
     float *h_velp;                   /* padded raw velocity                       */
-    long nsr, nst, nsp;
+    unsigned nsr, nst, nsp;
     float tdr, dr2, tdt, dt2, dp2;
 } spher_state;
 
@@ -154,11 +154,15 @@ int awefd__spher_create(awefd_model *m)
         return awefd__fail(AWEFD_EINVAL, "a %dx%d plane is too large for the step kernel's window", n1, n2);
     HIPCHK(hipMalloc((void **)&s->dS, sizeof(float) * (size_t)m->nalloc));
 
-    /* shell groups, as k_spher_shell enumerates them */
+    /* shell groups, as k_spher_shell enumerates them (32-bit node numbers) */
     const int cr = face_count(n1), ct = face_count(n2);
-    s->nsr = (long)cr * n2 * n3;
-    s->nst = (long)(n1 - cr) * ct * n3;
-    s->nsp = m->ndim == 3 ? (long)(n1 - cr) * (n2 - ct) * face_count(n3) : 0;
+    const long nsr = (long)cr * n2 * n3, nst = (long)(n1 - cr) * ct * n3;
+    const long nsp = m->ndim == 3 ? (long)(n1 - cr) * (n2 - ct) * face_count(n3) : 0;
+    if (nsr + nst + nsp >= (1L << 31) - 256)
+        return awefd__fail(AWEFD_EINVAL, "the %dx%dx%d grid has too many shell nodes", n1, n2, n3);
+    s->nsr = (unsigned)nsr;
+    s->nst = (unsigned)nst;
+    s->nsp = (unsigned)nsp;
 
     /* derivative divisors as the reference forms them (spher_utils.c:413-437) */
     const float dra = m->p.d[0], dth = m->p.d[1], dph = m->p.d[2];
@@ -476,6 +480,14 @@ static void spher_args(const awefd_model *m, awefd_spher_args *a)
     a->nsr = s->nsr;
     a->nst = s->nst;
     a->nsp = s->nsp;
+    {
+        const int cr = face_count(a->n1), ct = face_count(a->n2);
+        a->dv_rc = awefd_magic_div((unsigned)cr);
+        a->dv_n2 = awefd_magic_div((unsigned)a->n2);
+        a->dv_rin = awefd_magic_div((unsigned)(a->n1 - cr > 0 ? a->n1 - cr : 1));
+        a->dv_tc = awefd_magic_div((unsigned)ct);
+        a->dv_tin = awefd_magic_div((unsigned)(a->n2 - ct > 0 ? a->n2 - ct : 1));
+    }
     a->ninode = s->ninode;
     a->inode = s->d_inode;
     a->ibeg = s->d_ibeg;

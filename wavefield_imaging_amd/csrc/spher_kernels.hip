@@ -155,9 +155,11 @@ __device__ __forceinline__ long offs(const awefd_spher_args &a, node3 c)
  * uo(a) = um(n) + (um(a) - uo(n)) * b with n the inward neighbour of the
  * node's last rule; uo(n) is n's pre-boundary value S(n) for a low-face rule
  * and n's final value for a high-face rule.  The chain of high-face rules is
- * walked inward to its end, then evaluated outward in the reference's order. */
+ * walked inward to its end, then evaluated outward in the reference's order.
+ * This form walks the chain again for every level; shell_final below keeps
+ * the levels in registers and falls back to it for chains deeper than that. */
 template <int NDIM>
-__device__ float shell_final(const awefd_spher_args &a, node3 c0)
+__device__ __forceinline__ float shell_final_walk(const awefd_spher_args &a, node3 c0)
 {
     const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
     int rule = oneway_rule<NDIM>(c0.r, c0.t, c0.p, n1, n2, n3);
@@ -188,6 +190,53 @@ __device__ float shell_final(const awefd_spher_args &a, node3 c0)
     return x;
 }
 
+/* The same value, the chain's levels kept in registers: level l holds the
+ * current field at chain node c_l, at its inward neighbour c_(l+1) and the
+ * one-way coefficient of c_l's rule, all loaded on the way in (no load waits
+ * for another), then x = P(c_(l+1)) + (P(c_l) - x) b_l outward from the
+ * chain's end, the same operations in the same order.  Chains are at most 8
+ * deep, and on the 460 x 572 x 572 shell 99.8 % of the nodes have at most
+ * 2 levels (6.66 M none, 1.67 M one, 1.67 M two, 20 K deeper); a deeper
+ * chain takes the walk above. */
+constexpr int CHAIN_REG = 2;
+
+__device__ __forceinline__ bool high_rule(int r) { return r == 2 || r == 4 || r == 6; }
+
+template <int NDIM>
+__device__ __forceinline__ float shell_final(const awefd_spher_args &a, node3 c0)
+{
+    const int n1 = a.n1, n2 = a.n2, n3 = a.n3;
+    int rule = oneway_rule<NDIM>(c0.r, c0.t, c0.p, n1, n2, n3);
+    node3 c = c0;
+    float pc[CHAIN_REG], pn[CHAIN_REG], bc[CHAIN_REG];
+    float pcur = rule != 0 ? a.P[offs(a, c0)] : 0.f;
+    int depth = 0;
+#pragma unroll
+    for (int l = 0; l < CHAIN_REG; l++) {
+        if (!high_rule(rule)) break;
+        const node3 n = inward(c, rule);
+        pc[l] = pcur;
+        bc[l] = oneway_coef(a, c, rule);
+        pcur = a.P[offs(a, n)];
+        pn[l] = pcur;
+        c = n;
+        rule = oneway_rule<NDIM>(c.r, c.t, c.p, n1, n2, n3);
+        depth = l + 1;
+    }
+    if (high_rule(rule)) return shell_final_walk<NDIM>(a, c0);
+    float x;
+    if (rule == 0) {
+        x = a.S[offs(a, c)];
+    } else {
+        const node3 n = inward(c, rule);
+        x = a.P[offs(a, n)] + (pcur - a.S[offs(a, n)]) * oneway_coef(a, c, rule);
+    }
+#pragma unroll
+    for (int l = CHAIN_REG - 1; l >= 0; l--)
+        if (l < depth) x = pn[l] + (pc[l] - x) * bc[l];
+    return x;
+}
+
 /* The shell: every node within W layers of a face, enumerated as the r-face
  * slabs, then the theta-face slabs without r-face nodes, then the phi-face
  * slabs without either; r is the fastest index in each group. */
@@ -206,31 +255,36 @@ __device__ __forceinline__ span face_span(int n)
     return s;
 }
 
-template <int NDIM>
-__global__ __launch_bounds__(256) void k_spher_shell(const awefd_spher_args a)
+/* n / d for n < 2^31 by the multiplier the host prepared (awefd_magic_div) */
+__device__ __forceinline__ unsigned mdiv(unsigned n, awefd_udiv d)
 {
-    long t = (long)blockIdx.x * 256 + threadIdx.x;
+    return (__umulhi(n, d.m) + n) >> d.l;
+}
+
+/* Shell node number t (r-face group, then theta-face, then phi-face). */
+template <int NDIM>
+__device__ __forceinline__ void shell_node(const awefd_spher_args &a, unsigned t)
+{
     const span R = face_span(a.n1), T = face_span(a.n2), F = face_span(a.n3);
-    const int nr_in = a.n1 - R.count(), nt_in = a.n2 - T.count();
     node3 c;
     if (t < a.nsr) {
-        const int k = (int)(t % R.count());
-        const long q = t / R.count();
-        c.r = R.at(k);
-        c.t = (int)(q % a.n2);
-        c.p = (int)(q / a.n2);
+        const unsigned q = mdiv(t, a.dv_rc);            /* / R.count() */
+        const unsigned p = mdiv(q, a.dv_n2);
+        c.r = R.at((int)(t - q * (unsigned)R.count()));
+        c.t = (int)(q - p * (unsigned)a.n2);
+        c.p = (int)p;
     } else if ((t -= a.nsr) < a.nst) {
-        const int k = (int)(t % nr_in);
-        const long q = t / nr_in;
-        c.r = R.lo + k;
-        c.t = T.at((int)(q % T.count()));
-        c.p = (int)(q / T.count());
+        const unsigned q = mdiv(t, a.dv_rin);           /* / (n1 - R.count()) */
+        const unsigned p = mdiv(q, a.dv_tc);            /* / T.count() */
+        c.r = R.lo + (int)(t - q * (unsigned)(a.n1 - R.count()));
+        c.t = T.at((int)(q - p * (unsigned)T.count()));
+        c.p = (int)p;
     } else if (NDIM == 3 && (t -= a.nst) < a.nsp) {
-        const int k = (int)(t % nr_in);
-        const long q = t / nr_in;
-        c.r = R.lo + k;
-        c.t = T.lo + (int)(q % nt_in);
-        c.p = F.at((int)(q / nt_in));
+        const unsigned q = mdiv(t, a.dv_rin);
+        const unsigned p = mdiv(q, a.dv_tin);           /* / (n2 - T.count()) */
+        c.r = R.lo + (int)(t - q * (unsigned)(a.n1 - R.count()));
+        c.t = T.lo + (int)(q - p * (unsigned)(a.n2 - T.count()));
+        c.p = F.at((int)p);
     } else {
         return;
     }
@@ -246,21 +300,33 @@ __global__ __launch_bounds__(256) void k_spher_shell(const awefd_spher_args a)
     a.Q[offs(a, c)] = v;
 }
 
-/* Injection into the current field before the solve, one thread per node;
- * a node's contributions are added in the reference's source order. */
-__global__ __launch_bounds__(256) void k_spher_inject(const awefd_spher_args a)
+template <int NDIM>
+__global__ __launch_bounds__(256) void k_spher_shell(const awefd_spher_args a)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= a.ninode) return;
+    shell_node<NDIM>(a, blockIdx.x * 256u + threadIdx.x);
+}
+
+/* Injection of step it into field F at injection node i; a node's
+ * contributions are added in the reference's source order. */
+__device__ __forceinline__ void inject_node(const awefd_spher_args &a, float *F, int it, int i)
+{
     const long o = a.inode[i];
-    const float wa = a.wav[a.it];
-    float v = a.P[o];
+    const float wa = a.wav[it];
+    float v = F[o];
     for (int e = a.ibeg[i]; e < a.ibeg[i + 1]; e++) {
         float inc = wa * a.iw[2 * e];
         inc = inc * a.iw[2 * e + 1];   /* reflectivity (1 for constant injection) */
         v = v + inc;
     }
-    a.P[o] = v;
+    F[o] = v;
+}
+
+/* Injection into the current field before the solve, one thread per node. */
+__global__ __launch_bounds__(256) void k_spher_inject(const awefd_spher_args a)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.ninode) return;
+    inject_node(a, a.P, a.it, i);
 }
 
 /* Receiver gather from the finished field of the step. */
