@@ -98,6 +98,15 @@ int awefd_padded_dims(const awefd_model *m, int dims[3], int *nb);
  * MAWEFDspher2Dcpu.c:153-220, MAWEFDspher3Dcpu.c:185-272). */
 int awefd_set_model(awefd_model *m, const float *vel, const float *den);
 
+/* The velocity from device memory (d_vel: RSF order, unpadded, on the
+ * model's device, complete when called): the same padded (v dt)^2 and
+ * free-surface zeros as awefd_set_model(m, vel, NULL) (MAWEFDcart2Dcpu.c:
+ * 258-295 for a velocity that is already in HBM), so an inversion loop
+ * updates the model without a host round trip.  Cartesian models whose
+ * other tables do not depend on the velocity: not hybridbc=y, and in 3-D
+ * not dabc=y (AWEFD_EINVAL otherwise: use awefd_set_model). */
+int awefd_set_model_dev(awefd_model *m, const float *d_vel);
+
 /* Source / receiver coordinates as stored in the sou= / rec= files:
  * cart2d (x,z), cart3d (x,y,z), spher2d (r,theta), spher3d (r,phi,theta).
  * Replaces pt2dread1/pt3dread1 + lint2d_make/lint3d_make

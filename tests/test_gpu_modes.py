@@ -144,3 +144,18 @@ def test_sinc_taps_must_stay_on_the_grid(orc):
         with pytest.raises(awefd.L.AwefdError):
             m.set_sources(np.float32([[3.0, 90.0]]))      # 4 taps toward x < 0 leave the grid
         m.set_sources(np.float32([[0.0, 90.0]]))          # on a node: one tap
+
+
+@pytest.mark.parametrize("kw", [dict(hybridbc=True), dict(hybridbc=True, free=True), dict(adj=True, expl=True),
+                                dict(sinc=True, nb=7), dict(sinc=True, nb=7, adj=True)])
+def test_cart3d_modes_with_last_z_tile_in_rows(orc, kw):
+    """hybridbc, adj and sinc over grids whose last z tile runs as 16-lane rows
+    of 4 x tiles (padded z 72 = 64 + 8, or 74 with nb = 7), with a source
+    whose taps reach into that tile and snapshots."""
+    nz, nx, ny = 60, 34, 20
+    par, vel, sou, rec, wav = _case3d(nz=nz, nx=nx, ny=ny, nt=50, **kw)
+    sou = np.float32([[nx * 2.5 + 1.3, ny * 2.5 - 2.1, 6 * 5.0 + 0.7], [71.3, 52.9, (nz - 6) * 5.0 + 0.4]])
+    rec = np.concatenate([rec[::3], rec[::5] + np.float32([[0.7, -1.3, (nz - 9) * 5.0]])]).astype(np.float32)
+    k = 1 if kw.get("expl") else len(sou)
+    w = (wav[:, None] * np.float32([1.0, -0.6])[None, :k]).astype(np.float32)
+    _run_both("cart3d", par, vel, sou, rec, w if k > 1 else wav, orc, jsnap=19)

@@ -17,7 +17,7 @@ KIND_NAMES = {"cart2d": CART2D, "cart3d": CART3D, "spher2d": SPHER2D, "spher3d":
 
 # every symbol include/awefd.h declares
 EXPORTS = (
-    "awefd_params_default", "awefd_create", "awefd_padded_dims", "awefd_set_model",
+    "awefd_params_default", "awefd_create", "awefd_padded_dims", "awefd_set_model", "awefd_set_model_dev",
     "awefd_set_sources", "awefd_set_receivers", "awefd_set_wavelet", "awefd_run",
     "awefd_run_dev", "awefd_reset", "awefd_get_field", "awefd_debug_points",
     "awefd_profile", "awefd_kernel_stats", "awefd_device", "awefd_last_error",
@@ -69,6 +69,7 @@ def load():
         "awefd_create": (C.c_int, [P(Params), P(C.c_void_p)]),
         "awefd_padded_dims": (C.c_int, [C.c_void_p, P(C.c_int), P(C.c_int)]),
         "awefd_set_model": (C.c_int, [C.c_void_p, P(C.c_float), P(C.c_float)]),
+        "awefd_set_model_dev": (C.c_int, [C.c_void_p, C.c_void_p]),
         "awefd_set_sources": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float)]),
         "awefd_set_receivers": (C.c_int, [C.c_void_p, C.c_int, P(C.c_float)]),
         "awefd_set_wavelet": (C.c_int, [C.c_void_p, C.c_long, P(C.c_float)]),

@@ -16,8 +16,9 @@ checkpoints are live (two calls of one module before one backward, e.g.
 f(m(s1)) + f(m(s2))), it first runs its own forward again from the saved
 slowness, so every branch gets the gradient at its own model.
 
-The binding is ctypes over libawefd; torch provides device memory and the
-autograd graph only.  The library works on its own HIP stream and returns
+The model is set from device memory (awefd_set_model_dev), so an update of
+the slowness never passes through the host.  The binding is ctypes over
+libawefd; torch provides device memory and the autograd graph only.  The library works on its own HIP stream and returns
 after synchronising it; torch's current stream is synchronised before each
 call, so tensors torch produced are complete when the library reads them.
 """
@@ -37,9 +38,10 @@ def _forward_ckpt(owner, slow):
     """Set the model from squared slowness and run the checkpointed forward
     loop; returns the (nt, nr) traces."""
     m = owner.model
-    # the library takes velocity; v = s^(-1/2), formed in float64
-    vel = slow.detach().double().rsqrt().float().cpu().numpy()
-    m.set_model(vel)
+    # the library takes velocity; v = s^(-1/2), formed in float64 on the GPU
+    # and handed over in device memory
+    vel = slow.detach().double().rsqrt().float().contiguous()
+    m.set_model_dev(vel)
     tr = torch.empty((m.nt, m.nr), dtype=torch.float32, device=owner.device)
     torch.cuda.current_stream(owner.device).synchronize()
     L.check(m.lib.awefd_run_ckpt(m.h, _ptr(tr), int(owner.ckpt)))

@@ -102,6 +102,16 @@ class Model:
         den = None if den is None else L.f32(den)
         L.check(self.lib.awefd_set_model(self.h, L.fptr(vel), L.fptr(den) if den is not None else None))
 
+    def set_model_dev(self, vel):
+        """Velocity from a torch tensor on the model's GPU (RSF order,
+        unpadded, float32, contiguous): awefd_set_model_dev."""
+        import torch
+
+        if vel.dtype != torch.float32 or not vel.is_cuda or not vel.is_contiguous():
+            raise ValueError("set_model_dev: a contiguous float32 CUDA tensor")
+        torch.cuda.current_stream(vel.device).synchronize()
+        L.check(self.lib.awefd_set_model_dev(self.h, vel.data_ptr()))
+
     def set_sources(self, coords):
         c = L.f32(coords).reshape(-1)
         self.ns = c.size // self.ndim

@@ -368,6 +368,25 @@ static int cart_set_density(awefd_model *m, const float *den)
 }
 
 /* Cartesian model set-up: MAWEFDcart2Dcpu.c:258-295, MAWEFDcart3Dcpu.c:254-293. */
+/* sponge_make (fdutil.c) per axis: the factor of every padded node along
+ * each axis, 1 outside the strips */
+static int cart_sponge_tables(awefd_model *m)
+{
+    if (m->nb > 4096) return awefd__fail(AWEFD_EINVAL, "nb=%d too large", m->nb);
+    float w[4096];
+    awefd_sponge_weights(m->nb, w);
+    int rc = AWEFD_OK;
+    for (int a = 0; a < 3 && rc == 0; a++) {
+        free(m->h_sp[a]);
+        m->h_sp[a] = malloc(sizeof(float) * m->np[a]);
+        if (!m->h_sp[a]) return awefd__fail(AWEFD_ENOMEM, "sponge factors");
+        if (a < m->ndim) awefd_sponge_axis(w, m->nb, m->np[a], m->h_sp[a]);
+        else m->h_sp[a][0] = 1.f;
+        rc = upload(m, (void **)&m->dsp[a], m->h_sp[a], sizeof(float) * m->np[a]);
+    }
+    return rc;
+}
+
 static int cart_set_model(awefd_model *m, const float *vel, const float *den)
 {
     const int nbs[3] = {m->nb, m->nb, m->ndim == 3 ? m->nb : 0};
@@ -421,21 +440,7 @@ static int cart_set_model(awefd_model *m, const float *vel, const float *den)
         awefd_sponge_weights(m->nb - m->nop, w);
         rc = upload(m, (void **)&m->d_damp_h, w, sizeof(float) * (m->nb - m->nop));
     }
-    if (m->sponge && rc == 0) {
-        float w[4096];
-        awefd_sponge_weights(m->nb, w);
-        for (int a = 0; a < 3 && rc == 0; a++) {
-            free(m->h_sp[a]);
-            m->h_sp[a] = malloc(sizeof(float) * m->np[a]);
-            if (!m->h_sp[a]) {
-                rc = awefd__fail(AWEFD_ENOMEM, "sponge factors");
-                break;
-            }
-            if (a < m->ndim) awefd_sponge_axis(w, m->nb, m->np[a], m->h_sp[a]);
-            else m->h_sp[a][0] = 1.f;
-            rc = upload(m, (void **)&m->dsp[a], m->h_sp[a], sizeof(float) * m->np[a]);
-        }
-    }
+    if (m->sponge && rc == 0) rc = cart_sponge_tables(m);
     /* vel <- vel * (vel * dt^2) ; free surface zeroes z in [0, nb], except
      * under hybridbc (MAWEFDcart2Dcpu.c:289-295) */
     const float dt2 = m->p.dt * m->p.dt;
@@ -462,6 +467,40 @@ int awefd_set_model(awefd_model *m, const float *vel, const float *den)
     HIPCHK(hipSetDevice(m->dev));
     int rc = is_cart(m) ? cart_set_model(m, vel, den) : awefd__spher_set_model(m, vel);
     if (rc == 0) m->model_set = 1;
+    const int rd = setter_done(m);
+    return rc ? rc : rd;
+}
+
+/* The velocity from device memory: the same padded (v dt)^2 and free-surface
+ * zeros as awefd_set_model(m, vel, NULL), formed by a kernel, so a model
+ * update in an inversion loop does not pass through the host.  Only for the
+ * set-ups whose other tables do not depend on the velocity: cartesian, not
+ * hybridbc=y, and in 3-D not dabc=y (both build one-way boundary tables from
+ * the velocity on the host). */
+int awefd_set_model_dev(awefd_model *m, const float *d_vel)
+{
+    if (!m || !d_vel) return awefd__fail(AWEFD_EINVAL, "awefd_set_model_dev: NULL argument");
+    if (!is_cart(m))
+        return awefd__fail(AWEFD_EINVAL, "awefd_set_model_dev: cartesian models only (the spherical set-up "
+                                         "derives its one-way tables and reflectivities on the host)");
+    if ((m->sponge && m->ndim == 3) || m->hyb)
+        return awefd__fail(AWEFD_EINVAL, "awefd_set_model_dev: %s builds one-way boundary tables from the "
+                                         "velocity on the host; use awefd_set_model",
+                           m->hyb ? "hybridbc=y" : "3-D dabc=y");
+    HIPCHK(hipSetDevice(m->dev));
+    int rc = m->sponge ? cart_sponge_tables(m) : AWEFD_OK;
+    if (!rc) {
+        const float dt = m->p.dt;
+        const int nbs[3] = {m->nb, m->nb, m->ndim == 3 ? m->nb : 0};
+        const int e = awefd_launch_set_vel(d_vel, m->dvel, m->p.n, nbs, m->ld, dt * dt,
+                                           m->p.fsrf && !m->p.hybrid ? m->nb : -1, m->stream);
+        hipError_t he = e ? (hipError_t)e : hipStreamSynchronize(m->stream);
+        if (he != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "awefd_set_model_dev: %s", hipGetErrorString(he));
+    }
+    if (!rc) {
+        free_dev((void **)&m->drho);      /* constant density */
+        m->model_set = 1;
+    }
     const int rd = setter_done(m);
     return rc ? rc : rd;
 }

@@ -246,6 +246,11 @@ int awefd_launch_cart_abc_xy(const awefd_cart_args *a, int nop, void *stream);
  * 3-D y lines, each face of a line swept by one thread */
 int awefd_launch_cart_hybrid(const awefd_cart_args *a, int nop, int ndim, void *stream);
 int awefd_launch_fill(float *p, long n, float v, void *stream);
+/* dvel (padded, row stride ld) <- v (v dt2) of the edge-replicated model
+ * velocity d_vel (n[0] fastest), zero for z <= fsnb (fsnb < 0: no free
+ * surface) */
+int awefd_launch_set_vel(const float *d_vel, float *dvel, const int n[3], const int nbs[3], int ld, float dt2,
+                         int fsnb, void *stream);
 /* Batched 2-D step over many shots (cart_kernels.hip): shots = device array
  * of per-shot arguments, bshot = shot of every block, rx in {2,4,8,16}. */
 /* Batched 2-D steps it (and it+1 when two) of every shot in one pass:

@@ -1499,6 +1499,28 @@ __global__ __launch_bounds__(256) void k_cart_hybrid(const awefd_cart_args a, in
         for (int iz = 0; iz <= a.nb; iz++) a.P[b + iz] = 0.f;
 }
 
+/* the padded (v dt)^2 of awefd_set_model_dev: node (i1, i2, i3) takes model
+ * node (clamp(i1 - nb1), clamp(i2 - nb2), clamp(i3 - nb3)) (expand2d/3d,
+ * fdutil.c:435-571), then v (v dt2) as MAWEFDcart2Dcpu.c:289 forms it and
+ * the free surface's zeros (MAWEFDcart2Dcpu.c:291-295) */
+__global__ void k_set_vel(const float *vin, float *vout, int n1, int n2, int n3, int nb1, int nb2, int nb3, int ld,
+                          float dt2, int fsnb)
+{
+    const int p1 = n1 + 2 * nb1, p2 = n2 + 2 * nb2, p3 = n3 + 2 * nb3;
+    const long np = (long)p1 * p2 * p3;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (long)gridDim.x * blockDim.x) {
+        const int i1 = (int)(i % p1);
+        const long q = i / p1;
+        const int i2 = (int)(q % p2), i3 = (int)(q / p2);
+        const int k1 = min(max(i1 - nb1, 0), n1 - 1), k2 = min(max(i2 - nb2, 0), n2 - 1);
+        const int k3 = min(max(i3 - nb3, 0), n3 - 1);
+        const float v = vin[((long)k3 * n2 + k2) * n1 + k1];
+        float w = v * (v * dt2);
+        if (i1 <= fsnb) w = 0.f;
+        vout[((long)i3 * p2 + i2) * ld + i1] = w;
+    }
+}
+
 __global__ void k_fill(float *p, long n, float v)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -1670,6 +1692,16 @@ extern "C" int awefd_launch_cart_batch_t3(const awefd_cart_args *shots, const in
     case 6: return sse ? launch_t3_sse<6, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<6, false>(shots, bshot, nblk, it, roles, nst, rx, s);
     default: return (int)hipErrorInvalidValue;
     }
+}
+
+extern "C" int awefd_launch_set_vel(const float *d_vel, float *dvel, const int n[3], const int nbs[3], int ld,
+                                    float dt2, int fsnb, void *stream)
+{
+    const long np = (long)(n[0] + 2 * nbs[0]) * (n[1] + 2 * nbs[1]) * (n[2] + 2 * nbs[2]);
+    const unsigned g = (unsigned)min((np + 255) / 256, 65536L);
+    hipLaunchKernelGGL(k_set_vel, dim3(g), dim3(256), 0, (hipStream_t)stream, d_vel, dvel, n[0], n[1], n[2], nbs[0],
+                       nbs[1], nbs[2], ld, dt2, fsnb);
+    return (int)hipGetLastError();
 }
 
 extern "C" int awefd_launch_cart_batch_rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n,
