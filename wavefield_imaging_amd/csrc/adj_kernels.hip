@@ -227,14 +227,16 @@ __global__ __launch_bounds__(NWA * WAVE) void k_adj2d_step(const awefd_adj2d_arg
 }
 
 /* Every shot of a batch in one launch: block b belongs to shot bshot[b] and
- * is that shot's block b - blk0. */
+ * is that shot's block b - blk0.  Tiles of AWEFD_ADJ_RXB rows per wave (4:
+ * a batched gradient 4.5 % faster than with the single-grid kernel's 2; the
+ * single-grid loop is faster with 2). */
 template <int NOP>
 __global__ __launch_bounds__(NWA * WAVE) void k_adj2d_batch(const awefd_adj2d_args *__restrict__ shots,
                                                             const int *__restrict__ bshot, int n)
 {
     const int s = __builtin_amdgcn_readfirstlane(bshot[blockIdx.x]);
     const awefd_adj2d_args &a = shots[s];
-    adj_body<NOP, RXA>(a, (int)blockIdx.x - a.blk0, n);
+    adj_body<NOP, AWEFD_ADJ_RXB>(a, (int)blockIdx.x - a.blk0, n);
 }
 
 /* dJ/ds on the model grid, summed over the shots in shot order: for each

@@ -39,9 +39,10 @@ typedef struct {
     float *ck;                /* nck pairs (c_{kK}, c_{kK-1})             */
     float *seg;               /* K-1 fields c_{n0+1} .. c_{n0+K-1}        */
     float *A[3], *gV;         /* alpha fields in rotation, dJ/dV (padded) */
-    /* receiver and injection nodes bucketed per adjoint tile column */
-    awefd_inj *rinj, *sinj;
-    int *rbeg, *sbeg;
+    /* receiver and injection nodes bucketed per adjoint tile column: of the
+     * single-grid kernel's tiles, and (suffix b) of the batched kernel's */
+    awefd_inj *rinj, *sinj, *rinjb, *sinjb;
+    int *rbeg, *sbeg, *rbegb, *sbegb;
     unsigned lists_gen;
     int lists_ok;
     awefd_adj2d_args *d_args; /* device copy of the kernel arguments    */
@@ -51,8 +52,9 @@ typedef struct {
 static void free_lists(adj_state *s)
 {
     hipFree(s->rinj); hipFree(s->sinj); hipFree(s->rbeg); hipFree(s->sbeg);
-    s->rinj = s->sinj = NULL;
-    s->rbeg = s->sbeg = NULL;
+    hipFree(s->rinjb); hipFree(s->sinjb); hipFree(s->rbegb); hipFree(s->sbegb);
+    s->rinj = s->sinj = s->rinjb = s->sinjb = NULL;
+    s->rbeg = s->sbeg = s->rbegb = s->sbegb = NULL;
     s->lists_ok = 0;
 }
 
@@ -75,10 +77,10 @@ void awefd_adjoint_release(awefd_model *m)
 }
 
 /* Interpolation points of a 2-D grid as (node, id, weight) entries bucketed
- * per column of 64 x AWEFD_ADJ_TX tiles, uploaded to the device.  id is the
- * point index (receivers) or the wavelet column (sources: 0 with expl). */
+ * per column of 64 x tx tiles, uploaded to the device.  id is the point
+ * index (receivers) or the wavelet column (sources: 0 with expl). */
 static int point_buckets(const awefd_model *m, int n, const int *j, const float *w, int same_id, int ntz,
-                         int ntx, awefd_inj **d_inj, int **d_beg)
+                         int ntx, int tx, awefd_inj **d_inj, int **d_beg)
 {
     const long ne = 4L * n;
     const int nbucket = ntz * ntx;
@@ -97,7 +99,7 @@ static int point_buckets(const awefd_model *m, int n, const int *j, const float 
                 k->e.w = w[8 * a + q];
                 k->e.w2 = 1.f;
                 k->e.w3 = 1.f;
-                k->bucket = k->e.z / 64 + ntz * (k->e.x / AWEFD_ADJ_TX);
+                k->bucket = k->e.z / 64 + ntz * (k->e.x / tx);
             }
         awefd_sort_inj(keys, ne, nbucket, e, beg);
         hipError_t er = hipMalloc((void **)d_inj, sizeof(*e) * (ne > 0 ? ne : 1));
@@ -258,9 +260,13 @@ int awefd__adj_prepare(awefd_model *m)
     if (rc) return rc;
     if (!s->lists_ok || s->lists_gen != m->gen) {
         free_lists(s);
-        const int ntz = (m->np[0] + 63) / 64, ntx = (m->np[1] + AWEFD_ADJ_TX - 1) / AWEFD_ADJ_TX;
-        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntx, &s->rinj, &s->rbeg))) return rc;
-        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, m->p.expl, ntz, ntx, &s->sinj, &s->sbeg))) return rc;
+        const int ntz = (m->np[0] + 63) / 64, tx = AWEFD_ADJ_TX, txb = AWEFD_ADJ_TXB;
+        const int ntx = (m->np[1] + tx - 1) / tx, ntxb = (m->np[1] + txb - 1) / txb;
+        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntx, tx, &s->rinj, &s->rbeg))) return rc;
+        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, m->p.expl, ntz, ntx, tx, &s->sinj, &s->sbeg))) return rc;
+        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntxb, txb, &s->rinjb, &s->rbegb))) return rc;
+        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, m->p.expl, ntz, ntxb, txb, &s->sinjb, &s->sbegb)))
+            return rc;
         s->lists_ok = 1;
         s->lists_gen = m->gen;
     }
@@ -305,6 +311,19 @@ void awefd__adj_args(const awefd_model *m, const float *d_res, awefd_adj2d_args 
     g->sinj = s->sinj;
     g->wav = m->d_wav;
     g->wstride = m->p.expl ? 1 : m->ns;
+}
+
+/* The same for the batched kernel, whose tiles are 64 x AWEFD_ADJ_TXB. */
+void awefd__adj_args_batch(const awefd_model *m, const float *d_res, awefd_adj2d_args *g)
+{
+    const adj_state *s = state(m);
+    awefd__adj_args(m, d_res, g);
+    g->ntx = (m->np[1] + AWEFD_ADJ_TXB - 1) / AWEFD_ADJ_TXB;
+    g->nblk = g->ntz * g->ntx;
+    g->rbeg = s->rbegb;
+    g->rinj = s->rinjb;
+    g->sbeg = s->sbegb;
+    g->sinj = s->sinjb;
 }
 
 /* Checkpoint segment bases of the batched recompute. */

@@ -1065,7 +1065,7 @@ static int adj_tables(awefd_batch *b, const float *const *d_res)
     int *agnblk = calloc(ng, sizeof(int));
     int rc = (!ha || !agoff || !agnblk) ? awefd__fail(AWEFD_ENOMEM, "adjoint tables") : AWEFD_OK;
     for (int s = 0; s < ns && !rc; s++) {
-        awefd__adj_args(b->m[s], d_res[s], &ha[s]);
+        awefd__adj_args_batch(b->m[s], d_res[s], &ha[s]);
         ha[s].blk0 = agnblk[b->grp[s]];
         if ((long)agnblk[b->grp[s]] + ha[s].nblk > 0x7fffffffL) rc = awefd__fail(AWEFD_EINVAL, "adjoint: too many blocks");
         else agnblk[b->grp[s]] += ha[s].nblk;

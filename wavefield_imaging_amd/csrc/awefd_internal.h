@@ -125,6 +125,10 @@ typedef struct {
 #define AWEFD_ADJ_RX 2           /* x rows per wave of the adjoint step kernel   */
 #endif
 #define AWEFD_ADJ_TX (AWEFD_ADJ_NWAVE * AWEFD_ADJ_RX)
+#ifndef AWEFD_ADJ_RXB
+#define AWEFD_ADJ_RXB 4          /* x rows per wave of the batched adjoint kernel */
+#endif
+#define AWEFD_ADJ_TXB (AWEFD_ADJ_NWAVE * AWEFD_ADJ_RXB)
 typedef struct {
     /* per-step pointers follow from the step n: alpha_n lives in A[n % 3];
      * c_n is checkpoint k = n / K (ck + 2k nal) at n = kK, else segment

@@ -117,6 +117,7 @@ void   awefd__adj_mark(awefd_model *m);
 int    awefd__adj_ready(const awefd_model *m, const char *who);
 int    awefd__adj_prepare(awefd_model *m);
 void   awefd__adj_args(const awefd_model *m, const float *d_res, awefd_adj2d_args *g);
+void   awefd__adj_args_batch(const awefd_model *m, const float *d_res, awefd_adj2d_args *g);
 void   awefd__adj_cart_ckpt(const awefd_model *m, awefd_cart_args *a);
 void   awefd__adj_zero_jobs(const awefd_model *m, awefd_copy_job *jobs);
 
