@@ -307,6 +307,32 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True, fixed=No
     return out
 
 
+def grad_bytes_per_node(nt, K, depth):
+    """Algorithmic bytes per padded node of one shot's checkpointed gradient,
+    following the batch driver's schedule: the forward run (a single step
+    reads the previous and current field and (v dt)^2 and writes one field,
+    16 B; a three-step pass writes its last two fields, 20 B), the checkpoint
+    copies (two fields read and written, 16 B each), the segment recompute
+    (16 B a step; a three-step pass keeps all three fields, 24 B) and the
+    adjoint steps (28 B).  depth 3: passes while one ends inside its segment
+    (and, in the forward run, a step follows it), single steps after."""
+    fwd = rcp = 0
+    nck = (nt + K - 1) // K
+    for k in range(nck):
+        n0, n1 = k * K, min(k * K + K, nt)
+        it = n0
+        while depth == 3 and it + 3 <= n1 and it + 3 < nt:
+            fwd += 20
+            it += 3
+        fwd += 16 * (n1 - it)
+        n = n0
+        while depth == 3 and n + 3 <= n1 - 1:
+            rcp += 24
+            n += 3
+        rcp += 16 * max(n1 - 1 - n, 0)
+    return dict(forward=fwd, checkpoints=16 * nck, recompute=rcp, adjoint=28 * nt)
+
+
 def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cpu=False):
     """Misfit gradients w.r.t. squared slowness, 1000 x 2000 grid, nb=50,
     nt=4000, checkpoints every 64 steps, 2000 receivers per shot: the per-shot
@@ -365,7 +391,6 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cp
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     loop = sample / (t1 - t0)
-    bytes_grad = 60 * cells * nt     # forward 16 + recompute 16 + adjoint 28 B per node and step
     out = dict(workload="adjoint2d 1000x2000 nb=50 nt=%d ckpt=%d, %d shots x %d receivers, padded %s"
                         % (nt, K, nshot, nr, "x".join(map(str, m.padded[:2]))),
                per_shot_loop_gradients_per_s=round(loop, 3),
@@ -402,6 +427,11 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cp
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         bat = nshot / (t1 - t0)
+        # the batch driver's passes: three steps where the tile allows it
+        # (rx 4 or 8) unless AWEFD_CKPT_DEPTH=1 asks for one
+        depth = 3 if rx in (4, 8) and os.environ.get("AWEFD_CKPT_DEPTH") != "1" else 1
+        parts = grad_bytes_per_node(nt, K, depth)
+        bytes_grad = sum(parts.values()) * cells
         achieved = bytes_grad * nshot / (t1 - t0) / 1e9
         out.update(batched_gradients_per_s=round(bat, 3), batched_s_per_gradient_sum=round(t1 - t0, 3),
                    ms_per_time_step=round((t1 - t0) / nshot / nt * 1e3, 5),
@@ -412,11 +442,15 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cp
                              "frac": round(achieved / HBM_PEAK_GBS, 4),
                              "traffic": _traffic_scaled("adjoint2d_traffic.json", bytes_grad * nshot),
                              "traffic_per": "one batched gradient of all shots",
-                             "kernel": "k_adj2d_batch<2> with k_cart_batch2 (forward, recompute)",
+                             "kernel": ("k_adj2d_batch<2> with k_cart_batch_t3 (forward; recompute, all three "
+                                        "fields kept)" if depth == 3 else
+                                        "k_adj2d_batch<2> with k_cart_batch2 (forward, recompute)"),
+                             "steps_per_pass": depth,
                              "bytes_per_gradient": bytes_grad,
+                             "bytes_per_node": parts,
                              "timing": "the timed batched gradient (forward graph, residuals, backward graph): "
-                                       "60 B per node and step (forward 16, recompute 16, adjoint 28) over its "
-                                       "wall time"})
+                                       "the algorithmic bytes of its passes per padded node (bytes_per_node) "
+                                       "over its wall time"})
         b.close()
         for mm in ms:
             mm.close()

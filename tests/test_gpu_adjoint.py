@@ -7,6 +7,8 @@ finite differences of the GPU's own forward loop along 32 random smooth
 perturbations.  The forward run that keeps the checkpoints gives the traces
 of the plain run bit for bit, and the gradient does not depend on the
 checkpoint interval (bit for bit: recomputed fields are the forward's)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -235,8 +237,10 @@ def test_batched_gradient_is_the_sum_of_single_shots(orc):
         m = _model(par, vel, np.float32([[x, 45.0]]), rec, wav)
         ms.append(m)
         resl.append(rng.standard_normal((par["nt"], len(rec))).astype(np.float32))
+    single_f = []
     for m, r in zip(ms, resl):
         single_tr.append(_run_ckpt(m, 10).cpu().numpy())
+        single_f.append(m.field())
         single_g.append(_grad(m, r))
     gsum = single_g[0].copy()
     for g in single_g[1:]:
@@ -244,17 +248,31 @@ def test_batched_gradient_is_the_sum_of_single_shots(orc):
     b = awefd.Batch(ms)
     dres = [_dev(r) for r in resl]
     out = torch.empty(gsum.shape, dtype=torch.float32, device="cuda")
-    for rx, grp in ((8, 0), (4, 1)):          # 1 MiB groups: two groups of these shots
+    # tiles rx 4 and 8 run the checkpointed forward and the recompute in
+    # three-step passes (segments of 10, 8 and 4 steps end on one or two
+    # single steps, or on none); rx 16 and AWEFD_CKPT_DEPTH=1 one step per
+    # launch; 1 MiB groups: two groups of these shots
+    cases = ((8, 0, 10, None), (4, 1, 8, None), (8, 0, 4, None), (16, 0, 10, None), (8, 1, 10, "1"))
+    for rx, grp, K, depth in cases:
         b.set_tile(rx, grp)
-        for rep in range(2):
-            print("batched gradient rx %d group %d replay %d" % (rx, grp, rep), flush=True)
-            b.run_ckpt(10)
-            for s, m in enumerate(ms):
-                np.testing.assert_array_equal(m.traces(), single_tr[s], err_msg="shot %d" % s)
-            out.fill_(0.0)
-            torch.cuda.synchronize()
-            b.adjoint_grad([t.data_ptr() for t in dres], out.data_ptr())
-            np.testing.assert_array_equal(out.cpu().numpy(), gsum)
+        if depth is None:
+            os.environ.pop("AWEFD_CKPT_DEPTH", None)
+        else:
+            os.environ["AWEFD_CKPT_DEPTH"] = depth
+        try:
+            for rep in range(2):
+                print("batched gradient rx %d group %d ckpt %d depth %s replay %d" % (rx, grp, K, depth, rep),
+                      flush=True)
+                b.run_ckpt(K)
+                for s, m in enumerate(ms):
+                    np.testing.assert_array_equal(m.traces(), single_tr[s], err_msg="shot %d" % s)
+                    np.testing.assert_array_equal(m.field(), single_f[s], err_msg="final field, shot %d" % s)
+                out.fill_(0.0)
+                torch.cuda.synchronize()
+                b.adjoint_grad([t.data_ptr() for t in dres], out.data_ptr())
+                np.testing.assert_array_equal(out.cpu().numpy(), gsum)
+        finally:
+            os.environ.pop("AWEFD_CKPT_DEPTH", None)
     g64 = None
     for x, r in zip(xs, resl):
         g, _, _ = orc.grad_f64(par, vel, np.float32([[x, 45.0]]), rec, wav, res=r)

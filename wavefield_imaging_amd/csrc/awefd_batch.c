@@ -37,6 +37,16 @@ static const int RX_CAND[4] = {2, 4, 8, 16};
 #define NGC 5
 static const int GROUP_CAND[NGC] = {0, 32, 64, 128, 192};   /* MiB of fields per group */
 
+/* Copies that end a multi-step run: a field left in the batch's extra pair
+ * goes back into the model's own buffer (the slot the other field does not
+ * occupy); fin_p, fin_c: the slots (0: u0, 1: u1) of the previous and
+ * current field afterwards. */
+typedef struct {
+    awefd_copy_job *d_jobs;
+    long *d_cbeg, nch;
+    int njob, fin_p, fin_c;
+} final_copies;
+
 struct awefd_batch {
     int nshot;
     awefd_model **m;
@@ -84,13 +94,13 @@ struct awefd_batch {
     awefd_rx3 **d_rx3;               /* and their records                        */
     int t2_ok;                       /* multi-step tables built for the tile     */
     int t2_depth;                    /* ... and for this depth                   */
-    awefd_copy_job *d_fjobs;         /* end of a depth-2 run: fields back into   */
-    long *d_fcbeg, fnch;             /* the models' own buffers                  */
-    int nfjob, fin_p, fin_c;
+    final_copies fin;                /* end of a multi-step run                  */
+    final_copies kfin;               /* end of a three-step checkpointed run     */
     double tune2_ms[4][NGC], tune3_ms[4][NGC];
     hipGraph_t gfwd, gbwd;
     hipGraphExec_t xfwd, xbwd;
     int fwd_ok, bwd_ok;
+    int fwd_depth, bwd_depth;        /* steps per pass the graphs were made for  */
     float *bwd_gout;
     unsigned *mgen;                  /* each model's generation when the tables
                                         were built (setters bump it)           */
@@ -143,8 +153,10 @@ void awefd_batch_destroy(awefd_batch *b)
     free(b->d_rbeg3);
     free(b->d_rx3);
     dfree(&b->d_bshot2);
-    dfree(&b->d_fjobs);
-    dfree(&b->d_fcbeg);
+    dfree(&b->fin.d_jobs);
+    dfree(&b->fin.d_cbeg);
+    dfree(&b->kfin.d_jobs);
+    dfree(&b->kfin.d_cbeg);
     free(b->goff2);
     free(b->gnblk2);
     for (int s = 0; s < b->nshot; s++) {
@@ -501,13 +513,17 @@ static int t3_receivers(const awefd_model *m, int sz, int sx, int ntz, int ntx, 
     return AWEFD_OK;
 }
 
-static int t2_build(awefd_batch *b)
+static int t2_build(awefd_batch *b, int depth)
 {
     int sz, sx, rows, halo;
-    if (!tn_tile(b, b->depth, &sz, &sx, &rows, &halo))
+    if (!tn_tile(b, depth, &sz, &sx, &rows, &halo))
         return awefd__fail(AWEFD_EINVAL, "%d steps per pass need tile rx 4 or 8 (and a tile wider than its halo)",
-                           b->depth);
-    const int nthr = (b->depth == 3 ? AWEFD_BATCH_NWAVE3 : AWEFD_BATCH_NWAVE) * 64, ns = b->nshot, ng = b->ngroup;
+                           depth);
+    /* every captured graph that launches the multi-step kernels holds the
+     * block table about to be replaced */
+    b->graph_ok = b->fwd_ok = b->bwd_ok = 0;
+    b->t2_ok = 0;
+    const int nthr = (depth == 3 ? AWEFD_BATCH_NWAVE3 : AWEFD_BATCH_NWAVE) * 64, ns = b->nshot, ng = b->ngroup;
     if (!b->e0) b->e0 = calloc(ns, sizeof(float *));
     if (!b->e1) b->e1 = calloc(ns, sizeof(float *));
     if (!b->d_inj2) b->d_inj2 = calloc(ns, sizeof(awefd_inj *));
@@ -542,7 +558,7 @@ static int t2_build(awefd_batch *b)
         awefd_inj *inj = NULL;
         int *ibeg = NULL;
         long ninj = 0;
-        if (!rc) rc = t2_buckets(b, m, b->depth, h->ntz2, h->ntx2, &inj, &ibeg, &ninj);
+        if (!rc) rc = t2_buckets(b, m, depth, h->ntz2, h->ntx2, &inj, &ibeg, &ninj);
         if (!rc) {
             dfree(&b->d_inj2[s]);
             dfree(&b->d_ibeg2[s]);
@@ -558,7 +574,7 @@ static int t2_build(awefd_batch *b)
         free(ibeg);
         h->rbeg3 = NULL;
         h->rx3 = NULL;
-        if (!rc && b->depth == 3) {
+        if (!rc && depth == 3) {
             int *rbeg = NULL;
             awefd_rx3 *rx = NULL;
             rc = t3_receivers(m, sz, sx, h->ntz2, h->ntx2, &rbeg, &rx);
@@ -598,7 +614,7 @@ static int t2_build(awefd_batch *b)
         b->goff2 = goff2;
         b->gnblk2 = gnblk2;
         b->t2_ok = 1;
-        b->t2_depth = b->depth;
+        b->t2_depth = depth;
     } else {
         free(goff2);
         free(gnblk2);
@@ -660,23 +676,22 @@ static int t2_steps(awefd_batch *b, int g, int nsteps, int *fp, int *fc)
     return tn_schedule(b, b->t2_depth, g, nsteps, fp, fc, 1);
 }
 
-/* After a depth-2 run the last two fields may sit in the batch's extra pair:
- * copies into the models' own buffers (the slot the other field does not
- * occupy) */
-static int t2_final_jobs(awefd_batch *b, int fp, int fc)
+/* After a multi-step run the last two fields (final roles fp, fc) may sit in
+ * the batch's extra pair: the copies into the models' own buffers */
+static int final_jobs(awefd_batch *b, int fp, int fc, final_copies *f)
 {
     int tp = fp, tc = fc;
     if (tp >= 2) tp = tc == 0 ? 1 : 0;
     if (tc >= 2) tc = tp == 0 ? 1 : 0;
-    b->fin_p = tp;
-    b->fin_c = tc;
+    f->fin_p = tp;
+    f->fin_c = tc;
     const int per = (fp >= 2) + (fc >= 2);
-    b->nfjob = per * b->nshot;
-    dfree(&b->d_fjobs);
-    dfree(&b->d_fcbeg);
+    f->njob = per * b->nshot;
+    dfree(&f->d_jobs);
+    dfree(&f->d_cbeg);
     if (!per) return AWEFD_OK;
-    awefd_copy_job *jobs = malloc(sizeof(*jobs) * b->nfjob);
-    long *cbeg = malloc(sizeof(long) * b->nfjob);
+    awefd_copy_job *jobs = malloc(sizeof(*jobs) * f->njob);
+    long *cbeg = malloc(sizeof(long) * f->njob);
     if (!jobs || !cbeg) {
         free(jobs); free(cbeg);
         return awefd__fail(AWEFD_ENOMEM, "final copies");
@@ -697,11 +712,11 @@ static int t2_final_jobs(awefd_batch *b, int fp, int fc)
             c += (n + 1023) / 1024;
         }
     }
-    b->fnch = c;
-    hipError_t e = hipMalloc((void **)&b->d_fjobs, sizeof(*jobs) * b->nfjob);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->d_fcbeg, sizeof(long) * b->nfjob);
-    if (e == hipSuccess) e = hipMemcpy(b->d_fjobs, jobs, sizeof(*jobs) * b->nfjob, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(b->d_fcbeg, cbeg, sizeof(long) * b->nfjob, hipMemcpyHostToDevice);
+    f->nch = c;
+    hipError_t e = hipMalloc((void **)&f->d_jobs, sizeof(*jobs) * f->njob);
+    if (e == hipSuccess) e = hipMalloc((void **)&f->d_cbeg, sizeof(long) * f->njob);
+    if (e == hipSuccess) e = hipMemcpy(f->d_jobs, jobs, sizeof(*jobs) * f->njob, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(f->d_cbeg, cbeg, sizeof(long) * f->njob, hipMemcpyHostToDevice);
     free(jobs);
     free(cbeg);
     if (e != hipSuccess) return awefd__fail(AWEFD_EHIP, "final copies: %s", hipGetErrorString(e));
@@ -789,7 +804,7 @@ static int time_choice(awefd_batch *b, int depth, int rx, int group_mb, int step
         const int k = (steps - 1) / depth > 0 ? (steps - 1) / depth : 1;
         steps = depth * k + 1 <= b->nt ? depth * k + 1 : b->nt;
     }
-    if (!rc && depth >= 2) rc = t2_build(b);
+    if (!rc && depth >= 2) rc = t2_build(b, depth);
     if (!rc) rc = batch_reset(b);
     /* warm the tables */
     for (int g = 0; g < b->ngroup && !rc && depth >= 2; g++) rc = t2_steps(b, g, 1, &fp, &fc);
@@ -868,18 +883,18 @@ static int batch_capture(awefd_batch *b)
     b->graph = NULL;
     int fp = 0, fc = 1, rc;
     if (b->depth >= 2) {
-        if ((!b->t2_ok || b->t2_depth != b->depth) && (rc = t2_build(b))) return rc;
+        if ((!b->t2_ok || b->t2_depth != b->depth) && (rc = t2_build(b, b->depth))) return rc;
         /* the final roles are those of any group's schedule */
         int i = 0, c = 1;
         tn_schedule(b, b->depth, 0, b->nt, &i, &c, 0);
-        if ((rc = t2_final_jobs(b, i, c))) return rc;
+        if ((rc = final_jobs(b, i, c, &b->fin))) return rc;
     }
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     rc = batch_reset(b);
     if (b->depth >= 2) {
         for (int g = 0; g < b->ngroup && !rc; g++) rc = t2_steps(b, g, b->nt, &fp, &fc);
-        if (!rc && b->nfjob) {
-            int e = awefd_launch_copy_jobs(b->d_fjobs, b->d_fcbeg, b->nfjob, b->fnch, b->stream);
+        if (!rc && b->fin.njob) {
+            int e = awefd_launch_copy_jobs(b->fin.d_jobs, b->fin.d_cbeg, b->fin.njob, b->fin.nch, b->stream);
             if (e) rc = awefd__fail(AWEFD_EHIP, "final copies: %s", hipGetErrorString((hipError_t)e));
         }
     } else if (!rc) {
@@ -913,8 +928,8 @@ int awefd_batch_run(awefd_batch *b)
     for (int s = 0; s < b->nshot; s++) {
         awefd_model *m = b->m[s];
         if (b->depth >= 2) {
-            m->dP = b->fin_p ? b->u1[s] : b->u0[s];
-            m->dC = b->fin_c ? b->u1[s] : b->u0[s];
+            m->dP = b->fin.fin_p ? b->u1[s] : b->u0[s];
+            m->dC = b->fin.fin_c ? b->u1[s] : b->u0[s];
         } else {
             m->dP = b->nt % 2 ? b->u1[s] : b->u0[s];
             m->dC = b->nt % 2 ? b->u0[s] : b->u1[s];
@@ -931,20 +946,92 @@ static long chunks(long n)
     return (n + 1023) / 1024;
 }
 
+/* Steps per pass of the checkpointed forward run and of the segment
+ * recompute: three where the tile allows it (rx 4 or 8), else one.
+ * AWEFD_CKPT_DEPTH=1 keeps one step per launch. */
+static int ckpt_depth(const awefd_batch *b)
+{
+    int sz, sx, rows, halo;
+    const char *e = getenv("AWEFD_CKPT_DEPTH");
+    if (e && atoi(e) == 1) return 1;
+    return tn_tile(b, 3, &sz, &sx, &rows, &halo) ? 3 : 1;
+}
+
+/* Launch the checkpoint copies of group g at checkpoint k */
+static int ckpt_copies(awefd_batch *b, int g, int k)
+{
+    const long t = (long)g * b->nck + k;
+    const int nj = (int)((t + 1 < (long)b->ngroup * b->nck ? b->joff[t + 1] : 2L * b->nshot * b->nck) - b->joff[t]);
+    if (nj <= 0) return 0;
+    return awefd_launch_copy_jobs(b->d_jobs + b->joff[t], b->d_cbeg + b->joff[t], nj, b->jnch[t], b->stream);
+}
+
+/* The checkpointed forward run of group g in passes of three steps: in each
+ * segment [kK, min((k+1)K, nt)) the checkpoint copies of k, then three-step
+ * passes while a pass ends inside the segment and at least one step of the
+ * run follows it (the launch after a pass extracts its last field's trace
+ * row), then single steps.  The fields rotate through the shot's four
+ * buffers as in tn_schedule; the roles of the previous and current field at
+ * each checkpoint go to ckp[k], ckc[k] (when not NULL), the final roles to
+ * *fp, *fc.  With launch set the copies and passes are issued. */
+static int ckpt_schedule3(awefd_batch *b, int g, int *ckp, int *ckc, int *fp, int *fc, int launch)
+{
+    int iP = 0, iC = 1;
+    for (int k = 0; k < b->nck; k++) {
+        const int n0 = k * b->K, n1 = n0 + b->K < b->nt ? n0 + b->K : b->nt;
+        if (ckp) {
+            ckp[k] = iP;
+            ckc[k] = iC;
+        }
+        int e = launch ? ckpt_copies(b, g, k) : 0;
+        if (e) return awefd__fail(AWEFD_EHIP, "checkpoint %d copies: %s", k, hipGetErrorString((hipError_t)e));
+        int it = n0;
+        for (; it + 3 <= n1 && it + 3 < b->nt; it += 3) {
+            int d[2], nd = 0;
+            for (int i = 0; i < 4; i++)
+                if (i != iP && i != iC) d[nd++] = i;
+            const int roles = iP | iC << 2 | d[0] << 4 | d[1] << 6;
+            if (launch && (e = awefd_launch_cart_batch_t3(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it,
+                                                          roles, 3, b->nop, b->sse, b->rx, b->stream)))
+                return awefd__fail(AWEFD_EHIP, "checkpointed 3-step pass %d: %s", it, hipGetErrorString((hipError_t)e));
+            iP = d[0];
+            iC = d[1];
+        }
+        for (; it < n1; it++) {
+            int d0 = 0;
+            while (d0 == iP || d0 == iC) d0++;
+            const int roles = iP | iC << 2 | d0 << 4 | d0 << 6;
+            if (launch && (e = awefd_launch_cart_batch_t3(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], it,
+                                                          roles, 1, b->nop, b->sse, b->rx, b->stream)))
+                return awefd__fail(AWEFD_EHIP, "checkpointed step %d: %s", it, hipGetErrorString((hipError_t)e));
+            iP = iC;
+            iC = d0;
+        }
+    }
+    *fp = iP;
+    *fc = iC;
+    return AWEFD_OK;
+}
+
 /* Checkpoint copy tables: for group g and checkpoint k (step kK), every shot
- * of g copies its current and previous field (buffers by step parity, as
- * k_cart_batch2 alternates them) into its checkpoint pair k. */
-static int ckpt_tables(awefd_batch *b)
+ * of g copies its current and previous field into its checkpoint pair k.
+ * One step per launch alternates the two buffers by step parity (as
+ * k_cart_batch2 does); three steps per pass rotate four buffers, whose roles
+ * at each checkpoint the schedule gives (the same for every group). */
+static int ckpt_tables(awefd_batch *b, int depth)
 {
     const int ng = b->ngroup, nck = b->nck;
     long njob = 2L * b->nshot * nck;
     awefd_copy_job *jobs = malloc(sizeof(*jobs) * njob);
     long *cbeg = malloc(sizeof(long) * njob);
     long *joff = malloc(sizeof(long) * ng * nck), *jnch = malloc(sizeof(long) * ng * nck);
-    if (!jobs || !cbeg || !joff || !jnch) {
-        free(jobs); free(cbeg); free(joff); free(jnch);
+    int *ckp = malloc(sizeof(int) * nck), *ckc = malloc(sizeof(int) * nck);
+    if (!jobs || !cbeg || !joff || !jnch || !ckp || !ckc) {
+        free(jobs); free(cbeg); free(joff); free(jnch); free(ckp); free(ckc);
         return awefd__fail(AWEFD_ENOMEM, "checkpoint tables");
     }
+    int fp = 0, fc = 1;
+    if (depth == 3) ckpt_schedule3(b, 0, ckp, ckc, &fp, &fc, 0);
     long j = 0;
     for (int g = 0; g < ng; g++)
         for (int k = 0; k < nck; k++) {
@@ -957,6 +1044,10 @@ static int ckpt_tables(awefd_batch *b)
                 float *ck = awefd__adj_ckpt(m, k);
                 const float *cur = it & 1 ? b->u0[s] : b->u1[s];
                 const float *prev = it & 1 ? b->u1[s] : b->u0[s];
+                if (depth == 3) {
+                    cur = b->h[s].F[ckc[k]];
+                    prev = b->h[s].F[ckp[k]];
+                }
                 jobs[j] = (awefd_copy_job){cur, ck, m->nalloc};
                 cbeg[j++] = c;
                 c += chunks(m->nalloc);
@@ -966,6 +1057,8 @@ static int ckpt_tables(awefd_batch *b)
             }
             jnch[g * nck + k] = c;
         }
+    free(ckp);
+    free(ckc);
     dfree(&b->d_jobs);
     dfree(&b->d_cbeg);
     hipError_t e = hipMalloc((void **)&b->d_jobs, sizeof(*jobs) * njob);
@@ -979,31 +1072,34 @@ static int ckpt_tables(awefd_batch *b)
     b->joff = joff;
     b->jnch = jnch;
     if (e != hipSuccess) return awefd__fail(AWEFD_EHIP, "checkpoint tables: %s", hipGetErrorString(e));
+    if (depth == 3) return final_jobs(b, fp, fc, &b->kfin);
     return AWEFD_OK;
 }
 
-static int capture_fwd(awefd_batch *b)
+static int capture_fwd(awefd_batch *b, int depth)
 {
-    if (b->xfwd && b->fwd_ok) return AWEFD_OK;
+    if (b->xfwd && b->fwd_ok && b->fwd_depth == depth) return AWEFD_OK;
     if (b->xfwd) hipGraphExecDestroy(b->xfwd);
     if (b->gfwd) hipGraphDestroy(b->gfwd);
     b->xfwd = NULL;
     b->gfwd = NULL;
-    int rc = ckpt_tables(b);
+    int rc = AWEFD_OK;
+    if (depth == 3 && (!b->t2_ok || b->t2_depth != 3)) rc = t2_build(b, 3);
+    if (!rc) rc = ckpt_tables(b, depth);
     if (rc) return rc;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     rc = batch_reset(b);
-    for (int g = 0; g < b->ngroup && !rc; g++)
+    for (int g = 0; g < b->ngroup && !rc && depth == 3; g++) {
+        int fp, fc;
+        rc = ckpt_schedule3(b, g, NULL, NULL, &fp, &fc, 1);
+    }
+    if (!rc && depth == 3 && b->kfin.njob) {
+        int e = awefd_launch_copy_jobs(b->kfin.d_jobs, b->kfin.d_cbeg, b->kfin.njob, b->kfin.nch, b->stream);
+        if (e) rc = awefd__fail(AWEFD_EHIP, "checkpointed run, final copies: %s", hipGetErrorString((hipError_t)e));
+    }
+    for (int g = 0; g < b->ngroup && !rc && depth == 1; g++)
         for (int it = 0; it < b->nt && !rc; it++) {
-            int e = 0;
-            if (it % b->K == 0) {
-                const long t = (long)g * b->nck + it / b->K;
-                const int nj = (int)((t + 1 < (long)b->ngroup * b->nck ? b->joff[t + 1] : 2L * b->nshot * b->nck)
-                                     - b->joff[t]);
-                if (nj > 0)
-                    e = awefd_launch_copy_jobs(b->d_jobs + b->joff[t], b->d_cbeg + b->joff[t], nj, b->jnch[t],
-                                               b->stream);
-            }
+            int e = it % b->K == 0 ? ckpt_copies(b, g, it / b->K) : 0;
             if (!e)
                 e = awefd_launch_cart_batch2(b->d_shots, b->d_bshot + b->goff[g], b->gnblk[g], it, b->nop, b->sse,
                                              b->rx, b->stream);
@@ -1019,6 +1115,7 @@ static int capture_fwd(awefd_batch *b)
     b->gfwd = gr;
     HIPCHK(hipGraphInstantiate(&b->xfwd, gr, NULL, NULL, 0));
     b->fwd_ok = 1;
+    b->fwd_depth = depth;
     return AWEFD_OK;
 }
 
@@ -1041,14 +1138,20 @@ int awefd_batch_run_ckpt(awefd_batch *b, int ckpt)
         return awefd__fail(AWEFD_ENOMEM, "batch checkpoint bases");
     for (int s = 0; s < b->nshot; s++)
         if (b->fwd_ck[s] != awefd__adj_ckpt(b->m[s], 0)) b->fwd_ok = 0;
-    if ((rc = capture_fwd(b))) return rc;
+    const int depth = ckpt_depth(b);
+    if ((rc = capture_fwd(b, depth))) return rc;
     for (int s = 0; s < b->nshot; s++) b->fwd_ck[s] = awefd__adj_ckpt(b->m[s], 0);
     HIPCHK(hipGraphLaunch(b->xfwd, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     for (int s = 0; s < b->nshot; s++) {
         awefd_model *m = b->m[s];
-        m->dP = b->nt % 2 ? b->u1[s] : b->u0[s];
-        m->dC = b->nt % 2 ? b->u0[s] : b->u1[s];
+        if (depth == 3) {
+            m->dP = b->kfin.fin_p ? b->u1[s] : b->u0[s];
+            m->dC = b->kfin.fin_c ? b->u1[s] : b->u0[s];
+        } else {
+            m->dP = b->nt % 2 ? b->u1[s] : b->u0[s];
+            m->dC = b->nt % 2 ? b->u0[s] : b->u1[s];
+        }
         m->steps_done = b->nt;
         awefd__adj_mark(m);
     }
@@ -1132,20 +1235,30 @@ static int adj_tables(awefd_batch *b, const float *const *d_res)
     return rc;
 }
 
-static int capture_bwd(awefd_batch *b, float *d_grad)
+/* The backward pass of every shot as one graph: zero fills, then per group
+ * the segments last to first, each recomputed from its checkpoint (three
+ * steps per pass at depth 3, while the pass's last field is one the segment
+ * keeps, then single steps) and walked back by the adjoint steps; last the
+ * shots' dJ/dV summed in shot order into d_grad. */
+static int capture_bwd(awefd_batch *b, float *d_grad, int depth)
 {
-    if (b->xbwd && b->bwd_ok && b->bwd_gout == d_grad) return AWEFD_OK;
+    if (b->xbwd && b->bwd_ok && b->bwd_gout == d_grad && b->bwd_depth == depth) return AWEFD_OK;
     if (b->xbwd) hipGraphExecDestroy(b->xbwd);
     if (b->gbwd) hipGraphDestroy(b->gbwd);
     b->xbwd = NULL;
     b->gbwd = NULL;
     int rc = AWEFD_OK;
+    if (depth == 3 && (!b->t2_ok || b->t2_depth != 3) && (rc = t2_build(b, 3))) return rc;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     int e = awefd_launch_copy_jobs(b->d_zjobs, b->d_zcbeg, 3 * b->nshot, b->znch, b->stream);
     for (int g = 0; g < b->ngroup && !e; g++)
         for (int k = b->nck - 1; k >= 0 && !e; k--) {
             const int n0 = k * b->K, n1 = n0 + b->K < b->nt ? n0 + b->K : b->nt;
-            for (int n = n0; n + 1 < n1 && !e; n++)
+            int n = n0;
+            for (; depth == 3 && n + 3 <= n1 - 1 && !e; n += 3)
+                e = awefd_launch_cart_batch_t3rc(b->d_shots, b->d_bshot2 + b->goff2[g], b->gnblk2[g], n, b->nop,
+                                                 b->sse, b->rx, b->stream);
+            for (; n + 1 < n1 && !e; n++)
                 e = awefd_launch_cart_batch_rc(b->d_shots, b->d_bshot + b->goff[g], b->gnblk[g], n, b->nop, b->sse,
                                                b->rx, b->stream);
             for (int n = n1 - 1; n >= n0 && !e; n--)
@@ -1164,6 +1277,7 @@ static int capture_bwd(awefd_batch *b, float *d_grad)
     HIPCHK(hipGraphInstantiate(&b->xbwd, gr, NULL, NULL, 0));
     b->bwd_ok = 1;
     b->bwd_gout = d_grad;
+    b->bwd_depth = depth;
     return AWEFD_OK;
 }
 
@@ -1187,7 +1301,7 @@ int awefd_batch_adjoint_grad(awefd_batch *b, const float *const *d_res, float *d
     /* the tables hold the residual pointers and the buffers just prepared;
      * the captured graph reads them when it runs */
     if ((rc = adj_tables(b, d_res))) return rc;
-    if ((rc = capture_bwd(b, d_grad))) return rc;
+    if ((rc = capture_bwd(b, d_grad, ckpt_depth(b)))) return rc;
     HIPCHK(hipGraphLaunch(b->xbwd, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return AWEFD_OK;

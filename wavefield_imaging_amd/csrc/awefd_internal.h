@@ -272,6 +272,11 @@ int awefd_launch_cart_batch_t3(const awefd_cart_args *shots, const int *bshot, i
  * place, no extraction), same tables as awefd_launch_cart_batch2 */
 int awefd_launch_cart_batch_rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n, int nop,
                                int sse, int rx, void *stream);
+/* recompute steps n, n+1, n+2 of every shot inside its checkpoint segment in
+ * one pass (out of place, all three fields kept, no extraction); the tables
+ * of awefd_launch_cart_batch_t3, and n + 3 stays inside the segment */
+int awefd_launch_cart_batch_t3rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n, int nop,
+                                 int sse, int rx, void *stream);
 int awefd_launch_cart_batch2(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int nop,
                              int sse, int rx, void *stream);
 /* resident blocks per CU of the cartesian step kernel (0 if unknown) */

@@ -1124,18 +1124,18 @@ __global__ __launch_bounds__(NW * WAVE) void k_cart_batch_t2(const awefd_cart_ar
  * that owns the receiver's first corner (rbeg3/rx3), the others by the
  * following passes' extraction blocks.  nst = 1 takes one step (writing D1)
  * with the same tiles, for the end of the loop. */
-template <int NOP, bool SSE, int RX, int NW>
-__device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bid, int it, int roles, int nst)
+template <int NOP, bool SSE, int RX, int NW, bool RC>
+__device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bid, int it, const float *Pin,
+                                                const float *Cin, float *D0, float *D1, float *D2, int nst)
 {
     constexpr int ROWS = NW * RX;
     constexpr int H = 2 * NOP;                    /* tile halo for three steps */
     constexpr int SZ = WAVE - 2 * H;
     constexpr int SX = ROWS - 2 * H;
     static_assert(SX > 0 && SZ > 0, "three-step tile too small for the stencil");
-    const float *Pin = a.F[roles & 3], *Cin = a.F[(roles >> 2) & 3];
-    float *D1 = a.F[(roles >> 4) & 3], *D2 = a.F[(roles >> 6) & 3];
     const int tid = threadIdx.x;
     if (bid >= a.nblk2) {
+        if (RC) return;                           /* a recompute writes no traces */
         /* trace rows it (current field) and it-1 (previous field) */
         const int r = (bid - a.nblk2) * (NW * WAVE) + tid;
         cart_extract<2>(a, Cin, a.trow + (long)it * a.nr, r);
@@ -1190,8 +1190,9 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
     }
     const int col = tzi + a.ntz2 * txi;
     const int ib = a.ibeg2[col], ie = a.ibeg2[col + 1];
-    /* this tile's receivers for the in-block trace row, read early */
-    const int kb = a.rbeg3[col], ke = a.rbeg3[col + 1];
+    /* this tile's receivers for the in-block trace row, read early (none in
+     * a recompute) */
+    const int kb = RC ? 0 : a.rbeg3[col], ke = RC ? 0 : a.rbeg3[col + 1];
 
     /* xr[0]: the current field, then (once every wave is past the first
      * step) the second step's field; xr[1]: the first step's field */
@@ -1232,8 +1233,11 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
         float w = nv * spz;
         w = w * spx[j];
         s1[j] = w;
+        /* a recompute keeps every step's field: the first one too */
+        const int r = wv * RX + j;
+        if (RC && r >= H && r < ROWS - H) bst<NT>(mk_rsrc(D0, nbytes), vzo, (unsigned)(x0 + j) * rowb, w);
     }
-    if (nst == 1) {                               /* block-uniform */
+    if (!RC && nst == 1) {                        /* block-uniform */
 #pragma unroll
         for (int j = 0; j < RX; j++) {
             const int r = wv * RX + j;
@@ -1324,14 +1328,31 @@ __device__ __forceinline__ void cart_step3_body(const awefd_cart_args &a, int bi
     }
 }
 
-template <int NOP, bool SSE, int RX, int NW>
+/* RC: three steps it, it+1, it+2 of a checkpoint segment's recompute, out of
+ * place: from the segment's fields of steps it-1 and it (the checkpoint pair
+ * at the segment's start) into its buffers of steps it+1, it+2, it+3, which
+ * the host keeps inside the segment; the fields are those of the forward
+ * run, so no traces are written. */
+template <int NOP, bool SSE, int RX, int NW, bool RC>
 __global__ __launch_bounds__(NW * WAVE) void k_cart_batch_t3(const awefd_cart_args *__restrict__ shots,
                                                              const int *__restrict__ bshot, int it, int roles,
                                                              int nst)
 {
     const int s = __builtin_amdgcn_readfirstlane(bshot[blockIdx.x]);
     const awefd_cart_args &a = shots[s];
-    cart_step3_body<NOP, SSE, RX, NW>(a, (int)blockIdx.x - a.blk02, it, roles, nst);
+    const int bid = (int)blockIdx.x - a.blk02;
+    if (!RC) {
+        cart_step3_body<NOP, SSE, RX, NW, false>(a, bid, it, a.F[roles & 3], a.F[(roles >> 2) & 3], nullptr,
+                                                 a.F[(roles >> 4) & 3], a.F[(roles >> 6) & 3], nst);
+        return;
+    }
+    if (bid >= a.nblk2) return;
+    const int k = it / a.K, n0 = k * a.K;
+    const float *ckc = a.ck + 2L * k * a.nal;
+    const float *cur = it == n0 ? ckc : a.seg + (long)(it - n0 - 1) * a.nal;
+    const float *prev = it == n0 ? ckc + a.nal : (it - 1 == n0 ? ckc : a.seg + (long)(it - n0 - 2) * a.nal);
+    float *d = a.seg + (long)(it - n0) * a.nal;
+    cart_step3_body<NOP, SSE, RX, NW, true>(a, bid, it, prev, cur, d, d + a.nal, d + 2 * a.nal, 3);
 }
 
 /* z-high one-way ABC as a separate pass, for grids whose last z tile is too
@@ -1613,17 +1634,32 @@ int launch_t2_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int 
     return (int)hipGetLastError();
 }
 
-template <int NOP, bool SSE>
+template <int NOP, bool SSE, bool RC>
 int launch_t3_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int nst, int rx,
                   hipStream_t s)
 {
     constexpr int NW = AWEFD_BATCH_NWAVE3;
     switch (rx) {
-    case 4: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 4, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
-    case 8: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 8, NW>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
+    case 4: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 4, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
+    case 8: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 8, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
     default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();
+}
+
+template <bool RC>
+int launch_t3(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int nst, int nop, int sse,
+              int rx, hipStream_t s)
+{
+    switch (nop) {
+    case 1: return sse ? launch_t3_sse<1, true, RC>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<1, false, RC>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 2: return sse ? launch_t3_sse<2, true, RC>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<2, false, RC>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 3: return sse ? launch_t3_sse<3, true, RC>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<3, false, RC>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 4: return sse ? launch_t3_sse<4, true, RC>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<4, false, RC>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 5: return sse ? launch_t3_sse<5, true, RC>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<5, false, RC>(shots, bshot, nblk, it, roles, nst, rx, s);
+    case 6: return sse ? launch_t3_sse<6, true, RC>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<6, false, RC>(shots, bshot, nblk, it, roles, nst, rx, s);
+    default: return (int)hipErrorInvalidValue;
+    }
 }
 
 template <int NOP>
@@ -1682,16 +1718,13 @@ extern "C" int awefd_launch_cart_batch_t2(const awefd_cart_args *shots, const in
 extern "C" int awefd_launch_cart_batch_t3(const awefd_cart_args *shots, const int *bshot, int nblk, int it,
                                           int roles, int nst, int nop, int sse, int rx, void *stream)
 {
-    hipStream_t s = (hipStream_t)stream;
-    switch (nop) {
-    case 1: return sse ? launch_t3_sse<1, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<1, false>(shots, bshot, nblk, it, roles, nst, rx, s);
-    case 2: return sse ? launch_t3_sse<2, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<2, false>(shots, bshot, nblk, it, roles, nst, rx, s);
-    case 3: return sse ? launch_t3_sse<3, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<3, false>(shots, bshot, nblk, it, roles, nst, rx, s);
-    case 4: return sse ? launch_t3_sse<4, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<4, false>(shots, bshot, nblk, it, roles, nst, rx, s);
-    case 5: return sse ? launch_t3_sse<5, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<5, false>(shots, bshot, nblk, it, roles, nst, rx, s);
-    case 6: return sse ? launch_t3_sse<6, true>(shots, bshot, nblk, it, roles, nst, rx, s) : launch_t3_sse<6, false>(shots, bshot, nblk, it, roles, nst, rx, s);
-    default: return (int)hipErrorInvalidValue;
-    }
+    return launch_t3<false>(shots, bshot, nblk, it, roles, nst, nop, sse, rx, (hipStream_t)stream);
+}
+
+extern "C" int awefd_launch_cart_batch_t3rc(const awefd_cart_args *shots, const int *bshot, int nblk, int n,
+                                            int nop, int sse, int rx, void *stream)
+{
+    return launch_t3<true>(shots, bshot, nblk, n, 0, 3, nop, sse, rx, (hipStream_t)stream);
 }
 
 extern "C" int awefd_launch_set_vel(const float *d_vel, float *dvel, const int n[3], const int nbs[3], int ld,
