@@ -307,16 +307,20 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True, fixed=No
     return out
 
 
-def grad_bytes_per_node(nt, K, depth):
+def grad_bytes_per_node(nt, K, depth, adepth=2):
     """Algorithmic bytes per padded node of one shot's checkpointed gradient,
     following the batch driver's schedule: the forward run (a single step
     reads the previous and current field and (v dt)^2 and writes one field,
     16 B; a three-step pass writes its last two fields, 20 B), the checkpoint
     copies (two fields read and written, 16 B each), the segment recompute
     (16 B a step; a three-step pass keeps all three fields, 24 B) and the
-    adjoint steps (28 B).  depth 3: passes while one ends inside its segment
-    (and, in the forward run, a step follows it), single steps after."""
-    fwd = rcp = 0
+    adjoint steps (one step reads alpha_{n+1}, alpha_{n+2}, V, c_n and dJ/dV
+    and writes alpha_n and dJ/dV, 28 B; a two-step pass also reads c_{n-1}
+    and writes alpha_{n-1}, 36 B).  depth 3: passes while one ends inside its
+    segment (and, in the forward run, a step follows it), single steps after;
+    adepth 2: pairs of adjoint steps inside each segment, a single step for
+    an odd one left."""
+    fwd = rcp = adj = 0
     nck = (nt + K - 1) // K
     for k in range(nck):
         n0, n1 = k * K, min(k * K + K, nt)
@@ -330,7 +334,9 @@ def grad_bytes_per_node(nt, K, depth):
             rcp += 24
             n += 3
         rcp += 16 * max(n1 - 1 - n, 0)
-    return dict(forward=fwd, checkpoints=16 * nck, recompute=rcp, adjoint=28 * nt)
+        steps = n1 - n0
+        adj += 36 * (steps // 2) + 28 * (steps % 2) if adepth == 2 else 28 * steps
+    return dict(forward=fwd, checkpoints=16 * nck, recompute=rcp, adjoint=adj)
 
 
 def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cpu=False):
@@ -430,7 +436,8 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cp
         # the batch driver's passes: three steps where the tile allows it
         # (rx 4 or 8) unless AWEFD_CKPT_DEPTH=1 asks for one
         depth = 3 if rx in (4, 8) and os.environ.get("AWEFD_CKPT_DEPTH") != "1" else 1
-        parts = grad_bytes_per_node(nt, K, depth)
+        adepth = 1 if os.environ.get("AWEFD_ADJ_DEPTH") == "1" else 2
+        parts = grad_bytes_per_node(nt, K, depth, adepth)
         bytes_grad = sum(parts.values()) * cells
         achieved = bytes_grad * nshot / (t1 - t0) / 1e9
         out.update(batched_gradients_per_s=round(bat, 3), batched_s_per_gradient_sum=round(t1 - t0, 3),
@@ -442,10 +449,13 @@ def measure_grad(nt=4000, nshot=64, sample=8, batched=True, rx=8, group_mb=0, cp
                              "frac": round(achieved / HBM_PEAK_GBS, 4),
                              "traffic": _traffic_scaled("adjoint2d_traffic.json", bytes_grad * nshot),
                              "traffic_per": "one batched gradient of all shots",
-                             "kernel": ("k_adj2d_batch<2> with k_cart_batch_t3 (forward; recompute, all three "
-                                        "fields kept)" if depth == 3 else
-                                        "k_adj2d_batch<2> with k_cart_batch2 (forward, recompute)"),
+                             "kernel": "%s with %s" % (
+                                 "k_adj2d_batch2<2> (two adjoint steps per pass)" if adepth == 2 else
+                                 "k_adj2d_batch<2>",
+                                 "k_cart_batch_t3 (forward; recompute, all three fields kept)" if depth == 3 else
+                                 "k_cart_batch2 (forward, recompute)"),
                              "steps_per_pass": depth,
+                             "adjoint_steps_per_pass": adepth,
                              "bytes_per_gradient": bytes_grad,
                              "bytes_per_node": parts,
                              "timing": "the timed batched gradient (forward graph, residuals, backward graph): "

@@ -252,17 +252,21 @@ def test_batched_gradient_is_the_sum_of_single_shots(orc):
     # three-step passes (segments of 10, 8 and 4 steps end on one or two
     # single steps, or on none); rx 16 and AWEFD_CKPT_DEPTH=1 one step per
     # launch; 1 MiB groups: two groups of these shots
-    cases = ((8, 0, 10, None), (4, 1, 8, None), (8, 0, 4, None), (16, 0, 10, None), (8, 1, 10, "1"))
-    for rx, grp, K, depth in cases:
+    # adjoint: pairs of steps in one pass (segments of odd length end on a
+    # single step), or one step per launch with AWEFD_ADJ_DEPTH=1
+    cases = ((8, 0, 10, None, None), (4, 1, 8, None, None), (8, 0, 4, None, None), (16, 0, 10, None, None),
+             (8, 1, 10, "1", None), (8, 0, 10, None, "1"), (8, 0, 1, None, None))
+    for rx, grp, K, depth, adepth in cases:
         b.set_tile(rx, grp)
-        if depth is None:
-            os.environ.pop("AWEFD_CKPT_DEPTH", None)
-        else:
-            os.environ["AWEFD_CKPT_DEPTH"] = depth
+        for var, val in (("AWEFD_CKPT_DEPTH", depth), ("AWEFD_ADJ_DEPTH", adepth)):
+            if val is None:
+                os.environ.pop(var, None)
+            else:
+                os.environ[var] = val
         try:
             for rep in range(2):
-                print("batched gradient rx %d group %d ckpt %d depth %s replay %d" % (rx, grp, K, depth, rep),
-                      flush=True)
+                print("batched gradient rx %d group %d ckpt %d depth %s adjoint depth %s replay %d"
+                      % (rx, grp, K, depth, adepth, rep), flush=True)
                 b.run_ckpt(K)
                 for s, m in enumerate(ms):
                     np.testing.assert_array_equal(m.traces(), single_tr[s], err_msg="shot %d" % s)
@@ -273,12 +277,47 @@ def test_batched_gradient_is_the_sum_of_single_shots(orc):
                 np.testing.assert_array_equal(out.cpu().numpy(), gsum)
         finally:
             os.environ.pop("AWEFD_CKPT_DEPTH", None)
+            os.environ.pop("AWEFD_ADJ_DEPTH", None)
     g64 = None
     for x, r in zip(xs, resl):
         g, _, _ = orc.grad_f64(par, vel, np.float32([[x, 45.0]]), rec, wav, res=r)
         g64 = g if g64 is None else g64 + g
     print("  batched sum vs FP64 oracle %.3e" % _rel(gsum, g64), flush=True)
     assert _rel(gsum, g64) < GRAD_RTOL
+    b.close()
+    for m in ms:
+        m.close()
+
+
+@pytest.mark.parametrize("kw", CASES, ids=[str(c) or "default" for c in CASES])
+def test_batched_passes_equal_single_shots(orc, kw):
+    """Every gradient case through the batched passes (three-step forward
+    and recompute, two-step adjoint with dJ/dV carried across both steps,
+    whose overlapping tiles share the receiver and source nodes of their
+    rims): two shots, each one's traces and the sum of their gradients bit
+    for bit those of the single-shot loop."""
+    print("batched passes case", kw, flush=True)
+    par, vel, sou, rec, wav = _case(**kw)
+    sou2 = sou.copy()
+    sou2[:, 0] = par["n"][1] * 5.0 - sou2[:, 0]
+    ms = [_model(par, vel, s_, rec, wav) for s_ in (sou, sou2)]
+    rng = np.random.default_rng(6)
+    resl = [rng.standard_normal((par["nt"], len(rec))).astype(np.float32) for _ in ms]
+    tr1, g1 = [], None
+    for m, r in zip(ms, resl):
+        tr1.append(_run_ckpt(m, 7).cpu().numpy())
+        g = _grad(m, r)
+        g1 = g if g1 is None else g1 + g
+    b = awefd.Batch(ms)
+    b.set_tile(8, 0)
+    b.run_ckpt(7)
+    for s_, m in enumerate(ms):
+        np.testing.assert_array_equal(m.traces(), tr1[s_], err_msg="shot %d" % s_)
+    dres = [_dev(r) for r in resl]
+    out = torch.zeros(g1.shape, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    b.adjoint_grad([t.data_ptr() for t in dres], out.data_ptr())
+    np.testing.assert_array_equal(out.cpu().numpy(), g1)
     b.close()
     for m in ms:
         m.close()

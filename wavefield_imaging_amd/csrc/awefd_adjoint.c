@@ -9,9 +9,9 @@
  * checkpoint memory.  awefd_adjoint_grad visits the segments last to first:
  * it recomputes the fields of a segment from its checkpoint into K-1 segment
  * buffers (out-of-place steps, so the checkpoint survives and no copies are
- * made) and steps the adjoint backwards through them (three adjoint fields
+ * made) and steps the adjoint backwards through them (four adjoint fields
  * in rotation and the dJ/dV accumulator).  Fields held: 2 ceil(nt/K)
- * checkpoints + K-1 segment buffers + 4.
+ * checkpoints + K-1 segment buffers + 5.
  */
 #define __HIP_PLATFORM_AMD__ 1
 #include <hip/hip_runtime_api.h>
@@ -38,11 +38,12 @@ typedef struct {
     int valid;
     float *ck;                /* nck pairs (c_{kK}, c_{kK-1})             */
     float *seg;               /* K-1 fields c_{n0+1} .. c_{n0+K-1}        */
-    float *A[3], *gV;         /* alpha fields in rotation, dJ/dV (padded) */
+    float *A[4], *gV;         /* alpha fields in rotation, dJ/dV (padded) */
     /* receiver and injection nodes bucketed per adjoint tile column: of the
-     * single-grid kernel's tiles, and (suffix b) of the batched kernel's */
-    awefd_inj *rinj, *sinj, *rinjb, *sinjb;
-    int *rbeg, *sbeg, *rbegb, *sbegb;
+     * single-grid kernel's tiles, (suffix b) of the batched kernel's and
+     * (suffix 2) of the two-step pass's overlapping tiles */
+    awefd_inj *rinj, *sinj, *rinjb, *sinjb, *rinj2, *sinj2;
+    int *rbeg, *sbeg, *rbegb, *sbegb, *rbeg2, *sbeg2;
     unsigned lists_gen;
     int lists_ok;
     awefd_adj2d_args *d_args; /* device copy of the kernel arguments    */
@@ -53,8 +54,9 @@ static void free_lists(adj_state *s)
 {
     hipFree(s->rinj); hipFree(s->sinj); hipFree(s->rbeg); hipFree(s->sbeg);
     hipFree(s->rinjb); hipFree(s->sinjb); hipFree(s->rbegb); hipFree(s->sbegb);
-    s->rinj = s->sinj = s->rinjb = s->sinjb = NULL;
-    s->rbeg = s->sbeg = s->rbegb = s->sbegb = NULL;
+    hipFree(s->rinj2); hipFree(s->sinj2); hipFree(s->rbeg2); hipFree(s->sbeg2);
+    s->rinj = s->sinj = s->rinjb = s->sinjb = s->rinj2 = s->sinj2 = NULL;
+    s->rbeg = s->sbeg = s->rbegb = s->sbegb = s->rbeg2 = s->sbeg2 = NULL;
     s->lists_ok = 0;
 }
 
@@ -64,7 +66,7 @@ void awefd__adjoint_free(awefd_model *m)
     if (!s) return;
     hipSetDevice(m->dev);
     hipFree(s->ck); hipFree(s->seg);
-    hipFree(s->A[0]); hipFree(s->A[1]); hipFree(s->A[2]); hipFree(s->gV); hipFree(s->d_args);
+    hipFree(s->A[0]); hipFree(s->A[1]); hipFree(s->A[2]); hipFree(s->A[3]); hipFree(s->gV); hipFree(s->d_args);
     hipFree(s->ones);
     free_lists(s);
     free(s);
@@ -76,30 +78,53 @@ void awefd_adjoint_release(awefd_model *m)
     awefd__adjoint_free(m);
 }
 
-/* Interpolation points of a 2-D grid as (node, id, weight) entries bucketed
- * per column of 64 x tx tiles, uploaded to the device.  id is the point
- * index (receivers) or the wavelet column (sources: 0 with expl). */
-static int point_buckets(const awefd_model *m, int n, const int *j, const float *w, int same_id, int ntz,
-                         int ntx, int tx, awefd_inj **d_inj, int **d_beg)
+static int min_i(int a, int b)
 {
-    const long ne = 4L * n;
-    const int nbucket = ntz * ntx;
-    awefd_inj_key *keys = malloc(sizeof(*keys) * (ne > 0 ? ne : 1));
-    awefd_inj *e = malloc(sizeof(*e) * (ne > 0 ? ne : 1));
+    return a < b ? a : b;
+}
+
+/* Interpolation points of a 2-D grid as (node, id, weight) entries bucketed
+ * per tile, uploaded to the device.  id is the point index (receivers) or
+ * the wavelet column (sources: 0 with expl).  ov = 0: tiles of 64 x tx
+ * nodes, each node in one; ov = nop: the two-step pass's tiles, whose
+ * outputs are (64 - 2 ov) x (tx - 2 ov) nodes and whose extent reaches ov
+ * beyond them, each node in every tile whose extent holds it (at most two
+ * per axis). */
+static int point_buckets(const awefd_model *m, int n, const int *j, const float *w, int same_id, int ntz,
+                         int ntx, int tx, int ov, awefd_inj **d_inj, int **d_beg)
+{
+    const long cap = (ov ? 16L : 4L) * n;
+    const int nbucket = ntz * ntx, sz = 64 - 2 * ov, sx = tx - 2 * ov;
+    awefd_inj_key *keys = malloc(sizeof(*keys) * (cap > 0 ? cap : 1));
+    awefd_inj *e = malloc(sizeof(*e) * (cap > 0 ? cap : 1));
     int *beg = malloc(sizeof(int) * (nbucket + 1));
     int rc = (!keys || !e || !beg) ? awefd__fail(AWEFD_ENOMEM, "gradient point buckets") : AWEFD_OK;
+    long ne = 0;
     if (!rc) {
         for (int a = 0; a < n; a++)
             for (int q = 0; q < 4; q++) {
-                awefd_inj_key *k = &keys[4L * a + q];
-                k->e.z = j[3 * a] + (q & 1);
-                k->e.x = j[3 * a + 1] + ((q >> 1) & 1);
-                k->e.y = 0;
-                k->e.src = same_id ? 0 : a;
-                k->e.w = w[8 * a + q];
-                k->e.w2 = 1.f;
-                k->e.w3 = 1.f;
-                k->bucket = k->e.z / 64 + ntz * (k->e.x / tx);
+                awefd_inj en;
+                en.z = j[3 * a] + (q & 1);
+                en.x = j[3 * a + 1] + ((q >> 1) & 1);
+                en.y = 0;
+                en.src = same_id ? 0 : a;
+                en.w = w[8 * a + q];
+                en.w2 = 1.f;
+                en.w3 = 1.f;
+                if (!ov) {
+                    keys[ne].e = en;
+                    keys[ne++].bucket = en.z / 64 + ntz * (en.x / tx);
+                    continue;
+                }
+                /* tile t's extent: [t s - ov, t s - ov + width) */
+                const int vz = en.z + ov - 63, vx = en.x + ov - (tx - 1);
+                const int tz0 = vz <= 0 ? 0 : (vz + sz - 1) / sz, tz1 = min_i((en.z + ov) / sz, ntz - 1);
+                const int tx0 = vx <= 0 ? 0 : (vx + sx - 1) / sx, tx1 = min_i((en.x + ov) / sx, ntx - 1);
+                for (int bx = tx0; bx <= tx1; bx++)
+                    for (int bz = tz0; bz <= tz1; bz++) {
+                        keys[ne].e = en;
+                        keys[ne++].bucket = bz + ntz * bx;
+                    }
             }
         awefd_sort_inj(keys, ne, nbucket, e, beg);
         hipError_t er = hipMalloc((void **)d_inj, sizeof(*e) * (ne > 0 ? ne : 1));
@@ -237,6 +262,15 @@ done:
     return rc;
 }
 
+/* Tiles of the two-step pass: (64 - 2 nop) x (AWEFD_ADJ2_ROWS - 2 nop)
+ * outputs each. */
+static void adj2_tiles(const awefd_model *m, int *ntz2, int *ntx2)
+{
+    const int sz = 64 - 2 * m->nop, sx = AWEFD_ADJ2_ROWS - 2 * m->nop;
+    *ntz2 = (m->np[0] + sz - 1) / sz;
+    *ntx2 = (m->np[1] + sx - 1) / sx;
+}
+
 /* Buffers and point buckets of the backward pass (kept across calls). */
 int awefd__adj_prepare(awefd_model *m)
 {
@@ -245,7 +279,7 @@ int awefd__adj_prepare(awefd_model *m)
     int rc = AWEFD_OK;
     if (hipSetDevice(m->dev) != hipSuccess) return awefd__fail(AWEFD_EHIP, "hipSetDevice");
     if (s->K > 1) rc = dev_alloc(&s->seg, fb * (s->K - 1));
-    for (int i = 0; i < 3 && !rc; i++) rc = dev_alloc(&s->A[i], fb);
+    for (int i = 0; i < 4 && !rc; i++) rc = dev_alloc(&s->A[i], fb);
     if (!rc) rc = dev_alloc(&s->gV, fb);
     if (!rc && !m->p.dabc && !s->ones) {
         const long n = m->np[0] > m->np[1] ? m->np[0] : m->np[1];
@@ -262,10 +296,19 @@ int awefd__adj_prepare(awefd_model *m)
         free_lists(s);
         const int ntz = (m->np[0] + 63) / 64, tx = AWEFD_ADJ_TX, txb = AWEFD_ADJ_TXB;
         const int ntx = (m->np[1] + tx - 1) / tx, ntxb = (m->np[1] + txb - 1) / txb;
-        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntx, tx, &s->rinj, &s->rbeg))) return rc;
-        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, m->p.expl, ntz, ntx, tx, &s->sinj, &s->sbeg))) return rc;
-        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntxb, txb, &s->rinjb, &s->rbegb))) return rc;
-        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, m->p.expl, ntz, ntxb, txb, &s->sinjb, &s->sbegb)))
+        int ntz2, ntx2;
+        adj2_tiles(m, &ntz2, &ntx2);
+        const int ex = m->p.expl;
+        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntx, tx, 0, &s->rinj, &s->rbeg))) return rc;
+        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, ex, ntz, ntx, tx, 0, &s->sinj, &s->sbeg))) return rc;
+        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz, ntxb, txb, 0, &s->rinjb, &s->rbegb))) return rc;
+        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, ex, ntz, ntxb, txb, 0, &s->sinjb, &s->sbegb)))
+            return rc;
+        if ((rc = point_buckets(m, m->nr, m->h_rj, m->h_rw, 0, ntz2, ntx2, AWEFD_ADJ2_ROWS, m->nop, &s->rinj2,
+                                &s->rbeg2)))
+            return rc;
+        if ((rc = point_buckets(m, m->ns, m->h_sj, m->h_sw, ex, ntz2, ntx2, AWEFD_ADJ2_ROWS, m->nop, &s->sinj2,
+                                &s->sbeg2)))
             return rc;
         s->lists_ok = 1;
         s->lists_gen = m->gen;
@@ -278,7 +321,7 @@ void awefd__adj_args(const awefd_model *m, const float *d_res, awefd_adj2d_args 
 {
     const adj_state *s = state(m);
     memset(g, 0, sizeof *g);
-    for (int i = 0; i < 3; i++) g->A[i] = s->A[i];
+    for (int i = 0; i < 4; i++) g->A[i] = s->A[i];
     g->ck = s->ck;
     g->seg = s->seg;
     g->nal = m->nalloc;
@@ -311,6 +354,12 @@ void awefd__adj_args(const awefd_model *m, const float *d_res, awefd_adj2d_args 
     g->sinj = s->sinj;
     g->wav = m->d_wav;
     g->wstride = m->p.expl ? 1 : m->ns;
+    adj2_tiles(m, &g->ntz2, &g->ntx2);
+    g->nblk2 = g->ntz2 * g->ntx2;
+    g->rbeg2 = s->rbeg2;
+    g->rinj2 = s->rinj2;
+    g->sbeg2 = s->sbeg2;
+    g->sinj2 = s->sinj2;
 }
 
 /* The same for the batched kernel, whose tiles are 64 x AWEFD_ADJ_TXB. */
@@ -342,7 +391,7 @@ void awefd__adj_zero_jobs(const awefd_model *m, awefd_copy_job *jobs)
 {
     const adj_state *s = state(m);
     const int nt = m->p.nt;
-    float *z[3] = {s->A[nt % 3], s->A[(nt + 1) % 3], s->gV};
+    float *z[3] = {s->A[nt & 3], s->A[(nt + 1) & 3], s->gV};
     for (int i = 0; i < 3; i++) {
         jobs[i].src = NULL;
         jobs[i].dst = z[i];

@@ -78,6 +78,10 @@ struct awefd_batch {
     int *d_abshot;                   /* adjoint launch: block -> shot            */
     long *agoff;
     int *agnblk;
+    int *d_abshot2;                  /* two-step adjoint launch: block -> shot   */
+    long *agoff2;
+    int *agnblk2;
+    long abshot2_cap;
     awefd_copy_job *d_zjobs;         /* zero fills starting the backward pass    */
     long *d_zcbeg, znch;
     long abshot_cap;
@@ -101,6 +105,7 @@ struct awefd_batch {
     hipGraphExec_t xfwd, xbwd;
     int fwd_ok, bwd_ok;
     int fwd_depth, bwd_depth;        /* steps per pass the graphs were made for  */
+    int bwd_adepth;                  /* ... and adjoint steps per pass           */
     float *bwd_gout;
     unsigned *mgen;                  /* each model's generation when the tables
                                         were built (setters bump it)           */
@@ -128,6 +133,7 @@ void awefd_batch_destroy(awefd_batch *b)
     dfree(&b->d_cbeg);
     dfree(&b->d_adj);
     dfree(&b->d_abshot);
+    dfree(&b->d_abshot2);
     dfree(&b->d_zjobs);
     dfree(&b->d_zcbeg);
     free(b->joff);
@@ -135,6 +141,8 @@ void awefd_batch_destroy(awefd_batch *b)
     free(b->ha);
     free(b->agoff);
     free(b->agnblk);
+    free(b->agoff2);
+    free(b->agnblk2);
     free(b->grp);
     free(b->h);
     free(b->fwd_ck);
@@ -957,6 +965,14 @@ static int ckpt_depth(const awefd_batch *b)
     return tn_tile(b, 3, &sz, &sx, &rows, &halo) ? 3 : 1;
 }
 
+/* Adjoint steps per pass of the batched backward run: two (carrying dJ/dV
+ * across both), or one with AWEFD_ADJ_DEPTH=1. */
+static int adj_depth(void)
+{
+    const char *e = getenv("AWEFD_ADJ_DEPTH");
+    return e && atoi(e) == 1 ? 1 : 2;
+}
+
 /* Launch the checkpoint copies of group g at checkpoint k */
 static int ckpt_copies(awefd_batch *b, int g, int k)
 {
@@ -1164,20 +1180,31 @@ static int adj_tables(awefd_batch *b, const float *const *d_res)
 {
     const int ns = b->nshot, ng = b->ngroup;
     awefd_adj2d_args *ha = malloc(sizeof(*ha) * ns);
-    long *agoff = calloc(ng + 1, sizeof(long));
-    int *agnblk = calloc(ng, sizeof(int));
-    int rc = (!ha || !agoff || !agnblk) ? awefd__fail(AWEFD_ENOMEM, "adjoint tables") : AWEFD_OK;
+    long *agoff = calloc(ng + 1, sizeof(long)), *agoff2 = calloc(ng + 1, sizeof(long));
+    int *agnblk = calloc(ng, sizeof(int)), *agnblk2 = calloc(ng, sizeof(int));
+    int rc = (!ha || !agoff || !agnblk || !agoff2 || !agnblk2) ? awefd__fail(AWEFD_ENOMEM, "adjoint tables")
+                                                               : AWEFD_OK;
     for (int s = 0; s < ns && !rc; s++) {
         awefd__adj_args_batch(b->m[s], d_res[s], &ha[s]);
-        ha[s].blk0 = agnblk[b->grp[s]];
-        if ((long)agnblk[b->grp[s]] + ha[s].nblk > 0x7fffffffL) rc = awefd__fail(AWEFD_EINVAL, "adjoint: too many blocks");
-        else agnblk[b->grp[s]] += ha[s].nblk;
+        const int g = b->grp[s];
+        ha[s].blk0 = agnblk[g];
+        ha[s].blk02 = agnblk2[g];
+        if ((long)agnblk[g] + ha[s].nblk > 0x7fffffffL || (long)agnblk2[g] + ha[s].nblk2 > 0x7fffffffL)
+            rc = awefd__fail(AWEFD_EINVAL, "adjoint: too many blocks");
+        agnblk[g] += ha[s].nblk;
+        agnblk2[g] += ha[s].nblk2;
     }
-    for (int g = 0; g < ng && !rc; g++) agoff[g + 1] = agoff[g] + agnblk[g];
+    for (int g = 0; g < ng && !rc; g++) {
+        agoff[g + 1] = agoff[g] + agnblk[g];
+        agoff2[g + 1] = agoff2[g] + agnblk2[g];
+    }
     int *bshot = rc ? NULL : malloc(sizeof(int) * (agoff[ng] > 0 ? agoff[ng] : 1));
-    if (!rc && !bshot) rc = awefd__fail(AWEFD_ENOMEM, "adjoint block table");
-    for (int s = 0; s < ns && !rc; s++)
+    int *bshot2 = rc ? NULL : malloc(sizeof(int) * (agoff2[ng] > 0 ? agoff2[ng] : 1));
+    if (!rc && (!bshot || !bshot2)) rc = awefd__fail(AWEFD_ENOMEM, "adjoint block table");
+    for (int s = 0; s < ns && !rc; s++) {
         for (int k = 0; k < ha[s].nblk; k++) bshot[agoff[b->grp[s]] + ha[s].blk0 + k] = s;
+        for (int k = 0; k < ha[s].nblk2; k++) bshot2[agoff2[b->grp[s]] + ha[s].blk02 + k] = s;
+    }
     /* zero fills of every shot */
     awefd_copy_job *z = rc ? NULL : malloc(sizeof(*z) * 3 * ns);
     long *zc = rc ? NULL : malloc(sizeof(long) * 3 * ns);
@@ -1201,6 +1228,13 @@ static int adj_tables(awefd_batch *b, const float *const *d_res)
             e = hipMalloc((void **)&b->d_abshot, sizeof(int) * (agoff[ng] > 0 ? agoff[ng] : 1));
             if (e == hipSuccess) b->abshot_cap = agoff[ng];
         }
+        if (e == hipSuccess && b->abshot2_cap < agoff2[ng]) {
+            dfree(&b->d_abshot2);
+            b->abshot2_cap = 0;
+            b->bwd_ok = 0;
+            e = hipMalloc((void **)&b->d_abshot2, sizeof(int) * (agoff2[ng] > 0 ? agoff2[ng] : 1));
+            if (e == hipSuccess) b->abshot2_cap = agoff2[ng];
+        }
         if (e == hipSuccess && !b->d_adj) {
             b->bwd_ok = 0;
             e = hipMalloc((void **)&b->d_adj, sizeof(*ha) * ns);
@@ -1209,6 +1243,7 @@ static int adj_tables(awefd_batch *b, const float *const *d_res)
         }
         if (e == hipSuccess) e = hipMemcpy(b->d_adj, ha, sizeof(*ha) * ns, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(b->d_abshot, bshot, sizeof(int) * agoff[ng], hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(b->d_abshot2, bshot2, sizeof(int) * agoff2[ng], hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(b->d_zjobs, z, sizeof(*z) * 3 * ns, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(b->d_zcbeg, zc, sizeof(long) * 3 * ns, hipMemcpyHostToDevice);
         /* the recompute reads each shot's checkpoint and segment bases */
@@ -1217,20 +1252,27 @@ static int adj_tables(awefd_batch *b, const float *const *d_res)
         if (e != hipSuccess) rc = awefd__fail(AWEFD_EHIP, "adjoint tables: %s", hipGetErrorString(e));
     }
     free(bshot);
+    free(bshot2);
     free(z);
     free(zc);
     if (!rc) {
         free(b->ha);
         free(b->agoff);
         free(b->agnblk);
+        free(b->agoff2);
+        free(b->agnblk2);
         b->ha = ha;
         b->agoff = agoff;
         b->agnblk = agnblk;
+        b->agoff2 = agoff2;
+        b->agnblk2 = agnblk2;
         b->znch = nch;
     } else {
         free(ha);
         free(agoff);
         free(agnblk);
+        free(agoff2);
+        free(agnblk2);
     }
     return rc;
 }
@@ -1238,11 +1280,13 @@ static int adj_tables(awefd_batch *b, const float *const *d_res)
 /* The backward pass of every shot as one graph: zero fills, then per group
  * the segments last to first, each recomputed from its checkpoint (three
  * steps per pass at depth 3, while the pass's last field is one the segment
- * keeps, then single steps) and walked back by the adjoint steps; last the
- * shots' dJ/dV summed in shot order into d_grad. */
-static int capture_bwd(awefd_batch *b, float *d_grad, int depth)
+ * keeps, then single steps) and walked back by the adjoint steps (in pairs
+ * n, n-1 inside the segment at adepth 2, a single step for an odd one
+ * left); last the shots' dJ/dV summed in shot order into d_grad. */
+static int capture_bwd(awefd_batch *b, float *d_grad, int depth, int adepth)
 {
-    if (b->xbwd && b->bwd_ok && b->bwd_gout == d_grad && b->bwd_depth == depth) return AWEFD_OK;
+    if (b->xbwd && b->bwd_ok && b->bwd_gout == d_grad && b->bwd_depth == depth && b->bwd_adepth == adepth)
+        return AWEFD_OK;
     if (b->xbwd) hipGraphExecDestroy(b->xbwd);
     if (b->gbwd) hipGraphDestroy(b->gbwd);
     b->xbwd = NULL;
@@ -1261,8 +1305,13 @@ static int capture_bwd(awefd_batch *b, float *d_grad, int depth)
             for (; n + 1 < n1 && !e; n++)
                 e = awefd_launch_cart_batch_rc(b->d_shots, b->d_bshot + b->goff[g], b->gnblk[g], n, b->nop, b->sse,
                                                b->rx, b->stream);
-            for (int n = n1 - 1; n >= n0 && !e; n--)
-                e = awefd_launch_adj2d_batch(b->d_adj, b->d_abshot + b->agoff[g], b->agnblk[g], n, b->nop, b->stream);
+            int na = n1 - 1;
+            for (; adepth == 2 && na - 1 >= n0 && !e; na -= 2)
+                e = awefd_launch_adj2d_batch2(b->d_adj, b->d_abshot2 + b->agoff2[g], b->agnblk2[g], na, b->nop,
+                                              b->stream);
+            for (; na >= n0 && !e; na--)
+                e = awefd_launch_adj2d_batch(b->d_adj, b->d_abshot + b->agoff[g], b->agnblk[g], na, b->nop,
+                                             b->stream);
         }
     if (!e) e = awefd_launch_adj2d_chain(&b->ha[0], b->d_adj, b->nshot, d_grad, b->stream);
     if (e) rc = awefd__fail(AWEFD_EHIP, "batched backward pass: %s", hipGetErrorString((hipError_t)e));
@@ -1278,6 +1327,7 @@ static int capture_bwd(awefd_batch *b, float *d_grad, int depth)
     b->bwd_ok = 1;
     b->bwd_gout = d_grad;
     b->bwd_depth = depth;
+    b->bwd_adepth = adepth;
     return AWEFD_OK;
 }
 
@@ -1301,7 +1351,7 @@ int awefd_batch_adjoint_grad(awefd_batch *b, const float *const *d_res, float *d
     /* the tables hold the residual pointers and the buffers just prepared;
      * the captured graph reads them when it runs */
     if ((rc = adj_tables(b, d_res))) return rc;
-    if ((rc = capture_bwd(b, d_grad, ckpt_depth(b)))) return rc;
+    if ((rc = capture_bwd(b, d_grad, ckpt_depth(b), adj_depth()))) return rc;
     HIPCHK(hipGraphLaunch(b->xbwd, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
     return AWEFD_OK;

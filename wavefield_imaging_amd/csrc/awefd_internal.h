@@ -129,11 +129,24 @@ typedef struct {
 #define AWEFD_ADJ_RXB 4          /* x rows per wave of the batched adjoint kernel */
 #endif
 #define AWEFD_ADJ_TXB (AWEFD_ADJ_NWAVE * AWEFD_ADJ_RXB)
+/* The two-step adjoint pass: blocks of AWEFD_ADJ2_NWAVE waves x
+ * AWEFD_ADJ2_RX rows over 64 z; a tile's outputs are its inner
+ * (64 - 2 nop) x (rows - 2 nop) nodes.  4 x 8 (126 VGPRs, 4 waves per
+ * SIMD): the backward pass of 16 shots of 1100 x 2100 nodes 12 % faster
+ * than with one adjoint step per launch; 3 x 8 and 4 x 6 tie, 8 x 4 and
+ * 8 x 8 gain nothing, 6-wave and 16-wave blocks lose 10-30 %. */
+#ifndef AWEFD_ADJ2_NWAVE
+#define AWEFD_ADJ2_NWAVE 4
+#endif
+#ifndef AWEFD_ADJ2_RX
+#define AWEFD_ADJ2_RX 8
+#endif
+#define AWEFD_ADJ2_ROWS (AWEFD_ADJ2_NWAVE * AWEFD_ADJ2_RX)
 typedef struct {
-    /* per-step pointers follow from the step n: alpha_n lives in A[n % 3];
+    /* per-step pointers follow from the step n: alpha_n lives in A[n % 4];
      * c_n is checkpoint k = n / K (ck + 2k nal) at n = kK, else segment
      * buffer seg + (n - kK - 1) nal; residual row n at res + n nr */
-    float *A[3];
+    float *A[4];
     const float *ck, *seg;
     long nal;                /* floats per field                              */
     int K;
@@ -154,12 +167,22 @@ typedef struct {
     const float *wav;
     long wstride;
     int blk0;                /* batched: first block of this shot             */
+    /* the two-step pass: its tiles, the receiver and injection nodes of
+     * each tile's whole 64 x AWEFD_ADJ2_ROWS extent (overlapping tiles share
+     * nodes), and (batched) this shot's first block */
+    int ntz2, ntx2, nblk2, blk02;
+    const int *rbeg2, *sbeg2;
+    const awefd_inj *rinj2, *sinj2;
 } awefd_adj2d_args;
 
 int awefd_launch_adj2d_step(const awefd_adj2d_args *a, int n, int nop, void *stream);
 /* shots: device array of nshot argument records; bshot: block -> shot */
 int awefd_launch_adj2d_batch(const awefd_adj2d_args *shots, const int *bshot, int nblk, int n, int nop,
                              void *stream);
+/* backward steps n and n-1 of every shot in one pass (n-1 and n in one
+ * checkpoint segment); bshot: block -> shot of the two-step tiles */
+int awefd_launch_adj2d_batch2(const awefd_adj2d_args *shots, const int *bshot, int nblk, int n, int nop,
+                              void *stream);
 /* gout (model grid) = sum over the shots of dJ/ds; h: host copy of shot 0's
  * record (grid sizes), shots: device records */
 int awefd_launch_adj2d_chain(const awefd_adj2d_args *h, const awefd_adj2d_args *shots, int nshot, float *gout,
