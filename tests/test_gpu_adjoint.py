@@ -200,6 +200,44 @@ def test_torch_autograd_l2_misfit(orc):
     op.close()
 
 
+def test_torch_batch_traces(orc):
+    """BatchTraces: three shots through the batched graphs give each shot's
+    Traces output and, for any trace weights, the sum of the single-shot
+    gradients in shot order, bit for bit; loss.backward() agrees with the
+    direct gradient; a backward of an earlier forward call runs that call
+    again first."""
+    from wavefield_imaging_amd.autograd import BatchTraces, Traces
+
+    par, vel, sou, rec, wav = _case(nz=40, nx=56, nt=260)
+    srcs = [np.float32([[x, 45.0]]) for x in (60.0, 140.0, 220.0)]
+    op = BatchTraces(par, srcs, rec, wav, ckpt=16)
+    singles = [Traces(par, s_, rec, wav, ckpt=16) for s_ in srcs]
+    s0 = torch.tensor(1.0 / vel.astype(np.float64) ** 2, dtype=torch.float32, device="cuda")
+    w = torch.randn((len(srcs), par["nt"], len(rec)), device="cuda", generator=torch.Generator("cuda").manual_seed(3))
+    gsum = None
+    for k, op1 in enumerate(singles):
+        d1 = op1(s0)
+        g1 = op1.gradient(w[k])
+        gsum = g1 if gsum is None else gsum + g1
+        singles[k] = (op1, d1)
+    slow = s0.clone().requires_grad_(True)
+    d = op(slow)
+    assert tuple(d.shape) == (len(srcs), par["nt"], len(rec))
+    for k, (_, d1) in enumerate(singles):
+        np.testing.assert_array_equal(d[k].detach().cpu().numpy(), d1.cpu().numpy(), err_msg="shot %d" % k)
+    (d * w).sum().backward()
+    np.testing.assert_array_equal(slow.grad.cpu().numpy(), gsum.cpu().numpy())
+    np.testing.assert_array_equal(op.gradient(w).cpu().numpy(), gsum.cpu().numpy())
+    # two forward calls, then one backward through both
+    s1 = (s0 * 1.02).requires_grad_(True)
+    s2 = s0.clone().requires_grad_(True)
+    ((op(s2) * w).sum() + 0.0 * op(s1).sum()).backward()
+    np.testing.assert_array_equal(s2.grad.cpu().numpy(), gsum.cpu().numpy())
+    op.close()
+    for op1, _ in singles:
+        op1.close()
+
+
 def test_gradient_errors(orc):
     sh = _case()
     m = _model(*sh)
