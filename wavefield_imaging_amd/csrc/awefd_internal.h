@@ -27,7 +27,9 @@ extern "C" {
 #define AWEFD_COL_PD 3            /* planes its loads run ahead                    */
 #endif
 #define AWEFD_BATCH_NWAVE 4      /* waves per block of the batched 2-D kernel     */
+#ifndef AWEFD_BATCH_NWAVE3
 #define AWEFD_BATCH_NWAVE3 8     /* waves per block of the three-step 2-D pass    */
+#endif
 
 /* One source-injection contribution: node (z, x, y) += amp(src) * w * vel. */
 typedef struct {
