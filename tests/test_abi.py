@@ -123,3 +123,25 @@ def test_magic_division_is_exact():
         n = n[n <= np.uint64(top)]
         q = (((n * m) >> np.uint64(32)) + n) >> l
         assert np.array_equal(q, n // np.uint64(d)), d
+
+
+def test_gradient_schedule_bytes():
+    """The bench's byte count of a checkpointed gradient follows the batch
+    driver's schedule, counted here by hand: nt 157, segments of 10 (the last
+    of 7).  Forward: three passes and one single step per full segment, and
+    in the last (150..156) passes at 150 and 153, then the single step 156.
+    Recompute: three passes per full segment (u(n0+1..n0+9)), two in the last
+    (u(151..156)).  Adjoint: five pairs per full segment, three pairs and one
+    single step in the last."""
+    import bench
+
+    p = bench.grad_bytes_per_node(157, 10, 3, 2)
+    assert p["forward"] == 15 * (3 * 20 + 16) + (2 * 20 + 16)
+    assert p["recompute"] == 15 * 3 * 24 + 2 * 24
+    assert p["adjoint"] == 15 * 5 * 36 + 3 * 36 + 28
+    assert p["checkpoints"] == 16 * 16
+    one = bench.grad_bytes_per_node(157, 10, 1, 1)
+    assert one["forward"] == 16 * 157 and one["adjoint"] == 28 * 157
+    assert one["recompute"] == 16 * (157 - 16)          # every step but a segment's last
+    # segments of 64 on the 4000-step workload: 32.9 B per node and step
+    assert abs(sum(bench.grad_bytes_per_node(4000, 64, 3, 2).values()) / 4000 - 32.944) < 1e-9
