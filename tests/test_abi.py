@@ -145,3 +145,38 @@ def test_gradient_schedule_bytes():
     assert one["recompute"] == 16 * (157 - 16)          # every step but a segment's last
     # segments of 64 on the 4000-step workload: 32.9 B per node and step
     assert abs(sum(bench.grad_bytes_per_node(4000, 64, 3, 2).values()) / 4000 - 32.944) < 1e-9
+
+
+def test_exact_reciprocal_check():
+    """awefd__exact_recip accepts a divisor only if (float)((double)x * (1/c))
+    equals the IEEE float quotient x / c for every float x: checked here
+    exhaustively over the subnormal and smallest normal binades (where
+    quotients can be subnormal rounding midpoints) and one ordinary binade,
+    both signs; 98 and 150 have such midpoints that round the other way and
+    are refused."""
+    import numpy as np
+
+    lib = L.load()
+    f = lib.awefd__exact_recip
+    f.restype = C.c_double
+    f.argtypes = [C.c_float]
+    mant = np.arange(1 << 23, dtype=np.uint32)
+
+    def mismatches(c, rc):
+        bad = 0
+        for sign in (0, 1):
+            for e in (0, 1, 2, 3, 4, 5, 6, 7, 127):
+                x = ((np.uint32(sign) << 31) | (np.uint32(e) << 23) | mant).view(np.float32)
+                with np.errstate(all="ignore"):
+                    ref = x / np.float32(c)
+                    got = (x.astype(np.float64) * rc).astype(np.float32)
+                bad += int((ref.view(np.uint32) != got.view(np.uint32)).sum())
+        return bad
+
+    for c in (10.0, 25.0, 2.95115842e-06, 2.17733405e-12, 3.0, 12.0, 0.1):
+        rc = f(c)
+        assert rc != 0.0, c
+        assert mismatches(np.float32(c), rc) == 0, c
+    for c in (98.0, 150.0):
+        assert f(c) == 0.0, c
+        assert mismatches(np.float32(c), 1.0 / np.float64(np.float32(c))) > 0, c

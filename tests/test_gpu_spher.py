@@ -150,3 +150,28 @@ def test_spher3d_receivers_in_shell_and_on_sources(orc, which):
 def test_spher3d_source_in_shell(orc):
     par, vel, sou, rec, wav = _case(3, nb=4, nt=50, src=[[0.5, 14.0, 13.0]])
     _check("spher3d", par, vel, sou, rec, wav, orc)
+
+
+@pytest.mark.parametrize("case", ["reciprocal_products", "divisor_rejected", "ieee_forced"])
+def test_spher3d_derivative_divisions(orc, case, monkeypatch, capfd):
+    """The five derivative divisions by launch constants run as products
+    with the divisors' double reciprocals when the host has shown them to
+    round like the IEEE quotient for every float (dr = 5 m: 2 dr = 10 has
+    subnormal midpoint quotients, all rounded alike); dr = 49 m gives
+    2 dr = 98, for which some subnormal quotient rounds the other way, so
+    the kernel divides; AWEFD_SPHER_IEEEDIV=1 asks for the division.  Every
+    case is bit-exact against the oracle."""
+    monkeypatch.setenv("AWEFD_VERBOSE", "1")
+    if case == "ieee_forced":
+        monkeypatch.setenv("AWEFD_SPHER_IEEEDIV", "1")
+    par, vel, sou, rec, wav = _case(3, nt=60)
+    if case == "divisor_rejected":
+        d = list(par["d"])
+        d[0] = np.float32(49.0)
+        par["d"] = d
+        sou = _coords(par["o"], d, [[30 - 6.5, 14.4, 12.3]])
+        rec = _coords(par["o"], d, [[30 - 8.6, t, 13.3] for t in np.linspace(1.0, 26.0, 9)])
+    _check("spher3d", par, vel, sou, rec, wav, orc)
+    err = capfd.readouterr().err
+    want = "as double reciprocal products" if case == "reciprocal_products" else "by IEEE float division"
+    assert want in err, err

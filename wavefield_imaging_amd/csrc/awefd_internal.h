@@ -233,6 +233,10 @@ typedef struct {
     int ndim;
     /* derivative divisors, as the reference forms them in float */
     float tdr, dr2, tdt, dt2, dp2;  /* 2*dr, dr*dr, 2*dth, dth*dth, dph*dph      */
+    /* their reciprocals in double, for x / c = (float)((double)x * (1/c));
+     * fastdiv = 0: the kernels divide (awefd__exact_recip) */
+    double rtdr, rdr2, rtdt, rdt2, rdp2;
+    int fastdiv;
     /* 3-D metric: per r 2/r and 1/(r*r) (float); per (theta, r)
      * cos/((double)(r*r)*sin) and 1/((double)(r*r)*sin*sin) */
     const float *mr2, *mr1;

@@ -7,6 +7,7 @@
  */
 #include <math.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -117,6 +118,56 @@ awefd_udiv awefd_magic_div(unsigned d)
     r.l = l;
     r.m = (unsigned)(((1ULL << 32) * ((1ULL << l) - d)) / d + 1);
     return r;
+}
+
+/* 1/c in double such that (float)((double)x * (1/c)) is the IEEE float
+ * quotient x / c for every float x, or 0 when that does not hold for c.
+ *
+ * The double product differs from x / c by at most (2u + u^2) relative
+ * (u = 2^-53: one rounding of 1/c, one of the product).  A quotient q in the
+ * normal range is never a rounding midpoint mu of float (x = c mu would need
+ * 25 significant bits), and |q - mu| >= 2^(E-48) for the midpoints of its
+ * binade [2^E, 2^(E+1)): x - c mu is a non-zero multiple of 2^(ec+E-24) for
+ * c = C 2^ec, so dividing by c < 2^(ec+24) leaves at least 2^(E-48).  The
+ * product's error, below 2^(E-50.9), keeps it on q's side of every midpoint,
+ * and rounding it to float gives q's rounding.  Below 2^-126 the midpoints
+ * are the odd multiples of 2^-150; a subnormal quotient that is not one lies
+ * at least 2^min(ex, ec-150) / c from each (x = X 2^ex), more than the
+ * product's error (2^-177.9 when ex > ec - 150, else 2^(ex-ec-50.9)).  It
+ * can be one, though (x = 5 2^-149, c = 10), and then the
+ * product must round to that midpoint exactly for the ties-to-even rule to
+ * agree; those quotients are checked here one by one: every
+ * mu = (2m+1) 2^-150 below 2^-126 for which c mu (exact in double) is a
+ * float. */
+double awefd__exact_recip(float c)
+{
+    if (!(c > 0.0f) || !isfinite(c) || !isnormal(c)) return 0.0;
+    const double rc = 1.0 / (double)c;
+    /* c = C 2^ec with C odd; x = c mu = C (2m+1) 2^(ec-150) is a float when
+     * the odd C (2m+1) has at most 24 bits and, below 2^-126, when it is a
+     * multiple of 2^-149 */
+    int e;
+    uint64_t C = (uint64_t)ldexp(frexp((double)c, &e), 24);
+    int ec = e - 24;
+    while (!(C & 1)) {
+        C >>= 1;
+        ec++;
+    }
+    /* c a power of two: both forms are exact scalings.  ec < 1: a float x
+     * below 2^-126 is a multiple of 2^-149 and C (2m+1) 2^(ec-150) is not,
+     * and one above needs C (2m+1) >= 2^(24-ec) > 2^24 bits; no candidate. */
+    if (C == 1 || ec < 1) return rc;
+    for (uint64_t k = 1; C * k < (1u << 24); k += 2) {
+        const double xd = ldexp((double)(C * k), ec - 150);
+        const float x = (float)xd;                          /* exact */
+        const volatile float q = x / c;                     /* the IEEE quotient, on the host */
+        const float qv = q, f = (float)((double)x * rc);
+        uint32_t qb, fb;
+        memcpy(&qb, &qv, 4);
+        memcpy(&fb, &f, 4);
+        if (qb != fb) return 0.0;
+    }
+    return rc;
 }
 
 /* Spherical sponge factor, spher_utils.c:665-667 (double). */

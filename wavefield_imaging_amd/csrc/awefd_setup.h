@@ -21,6 +21,7 @@ void   awefd_lint(int ndim, int na, const float *coords, const int npad[3],
 float  awefd_oneway_cart(float v, float dt, float h);
 float  awefd_oneway_spher(float v, float dt, float h);
 void   awefd_sort_inj(awefd_inj_key *keys, long n, int nbucket, awefd_inj *out, int *ibeg);
+double awefd__exact_recip(float c);
 void   awefd_sinc_axis(float q, int zaxis, float w[9], int *f, int *n);
 int    awefd_sinc_points(int ndim, int na, const float *coords, const int npad[3], const float opad[3],
                          const float d[3], awefd_sinc_pt *out);

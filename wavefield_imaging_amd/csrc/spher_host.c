@@ -14,6 +14,7 @@
 #define __HIP_PLATFORM_AMD__ 1
 #include <hip/hip_runtime_api.h>
 #include <math.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -44,6 +45,8 @@ typedef struct {
     float *h_velp;                   /* padded raw velocity                       */
     unsigned nsr, nst, nsp;
     float tdr, dr2, tdt, dt2, dp2;
+    double rtdr, rdr2, rtdt, rdt2, rdp2;
+    int fastdiv;
 } spher_state;
 
 static spher_state *st(awefd_model *m) { return (spher_state *)m->spher; }
@@ -171,6 +174,17 @@ int awefd__spher_create(awefd_model *m)
     s->tdt = 2 * dth;
     s->dt2 = dth * dth;
     s->dp2 = dph * dph;
+    s->rtdr = awefd__exact_recip(s->tdr);
+    s->rdr2 = awefd__exact_recip(s->dr2);
+    s->rtdt = awefd__exact_recip(s->tdt);
+    s->rdt2 = awefd__exact_recip(s->dt2);
+    s->rdp2 = awefd__exact_recip(s->dp2);
+    const char *ev = getenv("AWEFD_SPHER_IEEEDIV");
+    s->fastdiv = s->rtdr != 0.0 && s->rdr2 != 0.0 && s->rtdt != 0.0 && s->rdt2 != 0.0 && s->rdp2 != 0.0 &&
+                 !(ev && atoi(ev) == 1);
+    if (getenv("AWEFD_VERBOSE"))
+        fprintf(stderr, "awefd: spherical derivatives %s\n",
+                s->fastdiv ? "as double reciprocal products" : "by IEEE float division");
 
     const float ora = m->opad[0], oth = m->opad[1];
     int rc = AWEFD_OK;
@@ -458,6 +472,12 @@ static void spher_args(const awefd_model *m, awefd_spher_args *a)
     a->tdt = s->tdt;
     a->dt2 = s->dt2;
     a->dp2 = s->dp2;
+    a->rtdr = s->rtdr;
+    a->rdr2 = s->rdr2;
+    a->rtdt = s->rtdt;
+    a->rdt2 = s->rdt2;
+    a->rdp2 = s->rdp2;
+    a->fastdiv = s->fastdiv;
     a->mr2 = s->d_mr2;
     a->mr1 = s->d_mr1;
     a->mt1 = s->d_mt1;

@@ -349,6 +349,18 @@ __global__ __launch_bounds__(256) void k_spher_extract(const awefd_spher_args a)
 }
 
 /* 2-D polar step: one thread per node (polar grids are small). */
+/* x / c for a launch divisor c.  FD: (float)((double)x * rc) with rc = 1/c in
+ * double, which the host has shown to give the IEEE quotient for every float
+ * x (awefd__exact_recip): one double multiplication and two conversions
+ * instead of the division's scale, reciprocal, four fused multiply-adds and
+ * fix-up. */
+template <bool FD>
+__device__ __forceinline__ float cdiv(float x, float c, double rc)
+{
+    if (FD) return (float)((double)x * rc);
+    return x / c;
+}
+
 __global__ __launch_bounds__(256) void k_spher_step2(const awefd_spher_args a)
 {
     const int n1 = a.n1, n2 = a.n2;
@@ -405,6 +417,7 @@ __device__ __forceinline__ void load_plane(float (&q)[RX], float (&v)[RX], float
 /* Six waves per SIMD: 75 VGPRs and no spill (the metric factors in LDS);
  * seven (72 VGPRs) or eight (64) spill and run 17 % and 50 % slower on the
  * 460 x 572 x 572 shell. */
+template <bool FD>
 __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args a)
 {
     constexpr int QN = 4;
@@ -507,11 +520,11 @@ __global__ __launch_bounds__(NTHR, 6) void k_spher_step3(const awefd_spher_args 
                 const float fm = q[j][R % QN];
                 const float fp = q[j][(R + 2) % QN];
                 /* solve_3D (spher_utils.c:412-448), every lane, then select */
-                const float pra = (rp - rm) / tdr;
-                const float ppra = ((rp - 2.f * c) + rm) / dr2;
-                const float pth = (tp - tm) / tdt;
-                const float ppth = ((tp - 2.f * c) + tm) / dt2;
-                const float ppph = ((fp - 2.f * c) + fm) / dp2;
+                const float pra = cdiv<FD>(rp - rm, tdr, a.rtdr);
+                const float ppra = cdiv<FD>((rp - 2.f * c) + rm, dr2, a.rdr2);
+                const float pth = cdiv<FD>(tp - tm, tdt, a.rtdt);
+                const float ppth = cdiv<FD>((tp - 2.f * c) + tm, dt2, a.rdt2);
+                const float ppph = cdiv<FD>((fp - 2.f * c) + fm, dp2, a.rdp2);
                 const float s1 = m2 * pra + ppra;
                 double acc = (double)s1 + mts[0][wv * RX + j][lane] * (double)pth;
                 acc = acc + (double)(m1 * ppth);
@@ -569,6 +582,7 @@ __device__ __forceinline__ void col_plane(float (&q)[CRX], float (&v)[CRX], floa
     hal[1] = bld(rP, vz, soff(x0 + CRX, p, pbase, rowb, psz));
 }
 
+template <bool FD>
 __global__ __launch_bounds__(WAVE) void k_spher_col3(const awefd_spher_args a)
 {
     extern __shared__ double col_dtab[];
@@ -666,11 +680,11 @@ __global__ __launch_bounds__(WAVE) void k_spher_col3(const awefd_spher_args a)
                 const float tp = j < CRX - 1 ? q[j + 1][C0] : sh_[B][1];
                 const float fm = q[j][(C0 + CQR - 1) % CQR];
                 const float fp = q[j][(C0 + 1) % CQR];
-                const float pra = (rp - rm) / tdr;
-                const float ppra = ((rp - 2.f * c) + rm) / dr2;
-                const float pth = (tp - tm) / tdt;
-                const float ppth = ((tp - 2.f * c) + tm) / dt2;
-                const float ppph = ((fp - 2.f * c) + fm) / dp2;
+                const float pra = cdiv<FD>(rp - rm, tdr, a.rtdr);
+                const float ppra = cdiv<FD>((rp - 2.f * c) + rm, dr2, a.rdr2);
+                const float pth = cdiv<FD>(tp - tm, tdt, a.rtdt);
+                const float ppth = cdiv<FD>((tp - 2.f * c) + tm, dt2, a.rdt2);
+                const float ppph = cdiv<FD>((fp - 2.f * c) + fm, dp2, a.rdp2);
                 const float s1 = m2 * pra + ppra;
                 double acc = (double)s1 + mt1[j] * (double)pth;
                 acc = acc + (double)(m1 * ppth);
@@ -708,9 +722,13 @@ extern "C" int awefd_launch_spher_step(const awefd_spher_args *a, void *stream)
 {
     hipStream_t s = (hipStream_t)stream;
     if (a->ndim == 3 && a->col) {
-        hipLaunchKernelGGL(k_spher_col3, dim3(a->nblk), dim3(WAVE), sizeof(double) * (size_t)(a->nb + 1), s, *a);
+        if (a->fastdiv)
+            hipLaunchKernelGGL(k_spher_col3<true>, dim3(a->nblk), dim3(WAVE), sizeof(double) * (size_t)(a->nb + 1), s, *a);
+        else
+            hipLaunchKernelGGL(k_spher_col3<false>, dim3(a->nblk), dim3(WAVE), sizeof(double) * (size_t)(a->nb + 1), s, *a);
     } else if (a->ndim == 3) {
-        hipLaunchKernelGGL(k_spher_step3, dim3(a->nblk), dim3(NTHR), 0, s, *a);
+        if (a->fastdiv) hipLaunchKernelGGL(k_spher_step3<true>, dim3(a->nblk), dim3(NTHR), 0, s, *a);
+        else hipLaunchKernelGGL(k_spher_step3<false>, dim3(a->nblk), dim3(NTHR), 0, s, *a);
     } else {
         dim3 g((a->n1 + 63) / 64, (a->n2 + 3) / 4);
         hipLaunchKernelGGL(k_spher_step2, g, dim3(256), 0, s, *a);
@@ -741,6 +759,6 @@ extern "C" int awefd_spher_step_occupancy(int ndim)
 {
     int nb = 0;
     if (ndim != 3) return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spher_step3, NTHR, 0) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_spher_step3<true>, NTHR, 0) != hipSuccess) return 0;
     return nb;
 }
