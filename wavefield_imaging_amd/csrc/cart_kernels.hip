@@ -831,28 +831,29 @@ __device__ __forceinline__ void cart_col_body(const awefd_cart_args &a, float *P
     const int xcd = bid & 7, slot = bid >> 3;
     const int xb = (int)((long)a.ntx * xcd / 8), nxk = (int)((long)a.ntx * (xcd + 1) / 8) - xb;
     if (nxk == 0) return;
-    int tzi, txi, tyi;
-    bool tail = false;
     if (!a.ztail) {
-        tzi = slot % a.ntz;
-        txi = xb + (slot / a.ntz) % nxk;
-        tyi = slot / (a.ntz * nxk);
-    } else {
-        /* groups of 4 x tiles: their 4 (ntz - 1) full z tiles, then one wave
-         * for the last z tile of all 4 (in the same launch: as a kernel of
-         * its own, launched after the full tiles, it cost more than it
-         * saved) */
-        const int ntzm = a.ntz - 1, gsz = 4 * ntzm + 1, per = ((nxk + 3) / 4) * gsz;
-        tyi = slot / per;
-        const int i = slot % per, g = i / gsz, r = i % gsz;
-        tail = r >= 4 * ntzm;
-        tzi = tail ? ntzm : r % ntzm;
-        txi = xb + 4 * g + (tail ? 0 : r / ntzm);
+        const int tzi = slot % a.ntz;
+        const int txi = xb + (slot / a.ntz) % nxk;
+        const int tyi = slot / (a.ntz * nxk);
+        if (tyi >= a.nty) return;
+        col_tile<NOP, SSE, VD, RX, PD, WAVE>(a, Pout, Cin, it, tzi, txi, tyi, 1, lds);
+        return;
     }
-    if (tyi >= a.nty || txi >= xb + nxk) return;
-    /* one call site per tile form: the tile body is inlined once each */
-    if (!tail) col_tile<NOP, SSE, VD, RX, PD, WAVE>(a, Pout, Cin, it, tzi, txi, tyi, 1, lds);
-    else col_tile<NOP, SSE, VD, RX, PD, 16>(a, Pout, Cin, it, tzi, txi, tyi, min(4, xb + nxk - txi), lds);
+    /* groups of 4 x tiles: their 4 (ntz - 1) full z tiles, then one wave for
+     * the last z tile of all 4 (in the same launch: as a kernel of its own,
+     * launched after the full tiles, it cost more than it saved) */
+    const int ntzm = a.ntz - 1, gsz = 4 * ntzm + 1, per = ((nxk + 3) / 4) * gsz;
+    const int tyi = slot / per;
+    if (tyi >= a.nty) return;
+    const int i = slot % per, g = i / gsz, r = i % gsz;
+    if (r < 4 * ntzm) {
+        const int txi = xb + 4 * g + r / ntzm;
+        if (txi >= xb + nxk) return;
+        col_tile<NOP, SSE, VD, RX, PD, WAVE>(a, Pout, Cin, it, r % ntzm, txi, tyi, 1, lds);
+    } else {
+        const int txi = xb + 4 * g;
+        col_tile<NOP, SSE, VD, RX, PD, 16>(a, Pout, Cin, it, ntzm, txi, tyi, min(4, xb + nxk - txi), lds);
+    }
 }
 
 template <int NOP, bool SSE, bool VD>
