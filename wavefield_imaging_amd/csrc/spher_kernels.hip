@@ -348,7 +348,6 @@ __global__ __launch_bounds__(256) void k_spher_extract(const awefd_spher_args a)
     a.trow[r] = s;
 }
 
-/* 2-D polar step: one thread per node (polar grids are small). */
 /* x / c for a launch divisor c.  FD: (float)((double)x * rc) with rc = 1/c in
  * double, which the host has shown to give the IEEE quotient for every float
  * x (awefd__exact_recip): one double multiplication and two conversions
@@ -361,6 +360,7 @@ __device__ __forceinline__ float cdiv(float x, float c, double rc)
     return x / c;
 }
 
+/* 2-D polar step: one thread per node (polar grids are small). */
 __global__ __launch_bounds__(256) void k_spher_step2(const awefd_spher_args a)
 {
     const int n1 = a.n1, n2 = a.n2;
