@@ -56,8 +56,8 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_UPDATE = 16
 
 WORKLOADS = {
-    "cart2d": dict(kernel="k_cart_step<2,2,true>",
-                   traffic=None,
+    "cart2d": dict(kernel="k_cart_step<2,2,true,false,true>",
+                   traffic="cart2d_step_traffic.json",
                    desc="cart2d 400x800 (z,x) h=5m nb=50 nt=%d dabc=y fdorder=4, padded 500x900, "
                         "1 source, 800 receivers"),
     "cart3d": dict(kernel="k_cart_col<2,true,false>",

@@ -47,6 +47,21 @@ def test_cart2d_bit_exact(orc, kw):
     _check("cart2d", *_case2d(**kw), orc)
 
 
+@pytest.mark.parametrize("narrow", ["0", "1"])
+@pytest.mark.parametrize("kw", [dict(), dict(fdorder=8), dict(free=True, sse=False), dict(dabc=False, nb=3)])
+def test_cart2d_both_tiles(orc, narrow, kw, monkeypatch, capfd):
+    """Both 2-D step tiles, forced: 4 waves x 4 x rows (the default once a
+    grid gives 3 such blocks per CU) and one wave x 3 rows (smaller grids).
+    151 padded x rows: neither 16 nor 3 divides the width."""
+    monkeypatch.setenv("AWEFD_VERBOSE", "1")
+    monkeypatch.setenv("AWEFD_NARROW", narrow)
+    par, vel, sou, rec, wav = _case2d(nz=70, nx=151 - 2 * kw.get("nb", 8), nt=90, **kw)
+    sou = np.array([[40.0, 45.0], [(par["n"][1] - 2) * 5.0, 300.0], [371.5, 12.0]], np.float32)
+    _check("cart2d", par, vel, sou, rec, np.stack([wav, -wav, 0.5 * wav], 1).astype(np.float32), orc)
+    err = capfd.readouterr().err
+    assert ("one-wave tiles" in err) == (narrow == "1"), err
+
+
 def test_cart2d_many_sources_expl(orc):
     par, vel, sou, rec, wav = _case2d(expl=True)
     sou = np.stack([np.arange(3, 50, 2) * 5.0 + 0.5, np.full(24, 12 * 5.0 + 1.2)], 1).astype(np.float32)

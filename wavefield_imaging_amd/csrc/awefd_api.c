@@ -107,11 +107,13 @@ void awefd_destroy(awefd_model *m)
 
 /* Tiling of the cartesian step kernel: z tiles of balanced width
  * <= 64 - 2*nop (a wave also holds nop halo lanes on each side), x tiles of
- * 16 rows, y chunks sized so that the launch has >= ~2048 blocks (8 per CU on
- * 256 CUs). */
+ * 16 rows (2-D grids too small for 3 such blocks per CU: 3 rows), y chunks
+ * sized so that the launch has >= ~2048 blocks (8 per CU on 256 CUs). */
 static void cart_tiling(awefd_model *m)
 {
     const int n1 = m->np[0], n2 = m->np[1], n3 = m->np[2];
+    int cus = 256;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, m->dev);
     if (m->target_blocks <= 0) {
         /* about 16K blocks (12-13 rounds of resident blocks on 256 CUs):
          * short y chunks keep neighbouring tiles in step, so halo rows are
@@ -119,8 +121,6 @@ static void cart_tiling(awefd_model *m)
          * AWEFD_ROUNDS=r asks for r whole rounds instead. */
         const char *e = getenv("AWEFD_TARGET_BLOCKS");
         const char *r = getenv("AWEFD_ROUNDS");
-        int cus = 256, dev = m->dev;
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         int per = awefd_cart_step_occupancy(m->nop, m->ndim, m->p.sse);
         if (per <= 0) per = 4;
         m->target_blocks = e ? atoi(e) : r ? atoi(r) * per * cus : 16384;
@@ -132,7 +132,11 @@ static void cart_tiling(awefd_model *m)
     m->zsep = m->sponge && m->ndim == 3 && m->ntz > 1 && n1 - 64 * (m->ntz - 1) < m->nop + 2;
     const char *ce = getenv("AWEFD_COL");
     m->col = m->ndim == 3 && !(ce && atoi(ce) == 0);
-    m->tx = m->col ? AWEFD_COL_RX : AWEFD_CART_TX;
+    /* AWEFD_NARROW=0/1 forces the 2-D tile */
+    const char *ne = getenv("AWEFD_NARROW");
+    const long wide = (long)m->ntz * ((n2 + AWEFD_CART_TX - 1) / AWEFD_CART_TX);
+    m->narrow = m->ndim == 2 && (ne ? atoi(ne) != 0 : wide < 3L * cus);
+    m->tx = m->col ? AWEFD_COL_RX : m->narrow ? AWEFD_CART_TX2 : AWEFD_CART_TX;
     m->ntx = (n2 + m->tx - 1) / m->tx;
     if (m->col) {
         /* one wave per (z tile, RX rows) column chunk of about
@@ -176,7 +180,9 @@ static void cart_tiling(awefd_model *m)
     if (m->ztail) m->nblk = 8 * (4 * (m->ntz - 1) + 1) * (((m->ntx + 7) / 8 + 3) / 4) * m->nty;
     if (getenv("AWEFD_VERBOSE"))
         fprintf(stderr, "awefd: step kernel%s tiles %d(z) x %d(x) x %d(y, <= %d planes) = %d blocks, "
-                "target %d\n", m->col ? (m->ztail ? " (columns, last z tile in 16-lane rows)" : " (columns)") : "",
+                "target %d\n",
+                m->col ? (m->ztail ? " (columns, last z tile in 16-lane rows)" : " (columns)")
+                       : m->narrow ? " (one-wave tiles)" : "",
                 m->ntz, m->ntx, m->nty, m->ychunk, m->nblk, m->target_blocks);
 }
 
@@ -859,6 +865,7 @@ void awefd__cart_args(const awefd_model *m, awefd_cart_args *a)
     a->ychunk = m->ychunk;
     a->nty = m->nty;
     a->col = m->col;
+    a->narrow = m->narrow;
     a->ztail = m->ztail;
     a->nblk = m->nblk;
     a->dabc = m->sponge;

@@ -20,6 +20,18 @@ extern "C" {
 #define AWEFD_NWAVE 4             /* waves per block of the cartesian step kernel */
 #endif
 #define AWEFD_CART_TX (AWEFD_NWAVE * AWEFD_RX) /* x rows per stencil block     */
+/* The narrow 2-D tile: one wave of 3 x rows per block, for grids whose
+ * 4 x 4 tiling leaves fewer than 3 blocks per CU.  On the 500 x 900 grid
+ * (456 blocks of 4 x 4) it runs 3.68 us per step against 4.34 us (1 wave x
+ * 2 rows 3.86, x 4 4.10, x 5 3.98, x 6 4.21; 2 waves x 2 rows 4.24); from
+ * 800 x 1440 (1170 blocks) on, 4 x 4 is 3-10 % faster. */
+#ifndef AWEFD_RX2
+#define AWEFD_RX2 3
+#endif
+#ifndef AWEFD_NWAVE2
+#define AWEFD_NWAVE2 1
+#endif
+#define AWEFD_CART_TX2 (AWEFD_NWAVE2 * AWEFD_RX2)
 #ifndef AWEFD_COL_RX
 #define AWEFD_COL_RX 2            /* x rows per wave of the 3-D column step kernel */
 #endif
@@ -77,6 +89,7 @@ typedef struct {
     /* tiling */
     int tz, ntz, ntx, ychunk, nty, nblk; /* ychunk: longest y chunk      */
     int col;                 /* 3-D: the column kernel (one wave per block)    */
+    int narrow;              /* 2-D: the one-wave tile of AWEFD_RX2 x rows     */
     int ztail;               /* column kernel: the last z tile (n1 % 64 <= 16
                                 nodes) as 16-lane rows, 4 x tiles a wave */
     /* sponge: per-axis factors, 1 outside the strips */

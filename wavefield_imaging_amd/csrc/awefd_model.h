@@ -63,6 +63,7 @@ struct awefd_model {
     int tz, ntz, ntx, ychunk, nty, nblk;
     int tx;                       /* x rows per step-kernel tile (bucket width) */
     int col;                      /* 3-D step in columns (k_cart_col)        */
+    int narrow;                   /* 2-D step in one-wave tiles              */
     int ztail;                    /* its last z tile in 16-lane rows         */
     int target_blocks;            /* stencil blocks the y chunking aims at   */
     int zsep;                     /* z-high one-way ABC as a separate pass   */

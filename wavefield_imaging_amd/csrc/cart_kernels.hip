@@ -59,6 +59,12 @@ constexpr int RX0 = AWEFD_RX;         /* x rows per wave of the single-grid kern
 #endif
 constexpr int NT = AWEFD_NT;
 static_assert(NWAVE0 * RX0 == AWEFD_CART_TX, "tile width");
+/* tile of the single-grid step kernel: the narrow 2-D one, or 4 waves */
+template <bool NARROW>
+constexpr int step_nw() { return NARROW ? AWEFD_NWAVE2 : NWAVE0; }
+template <bool NARROW>
+constexpr int step_rx() { return NARROW ? AWEFD_RX2 : RX0; }
+static_assert(AWEFD_NWAVE2 * AWEFD_RX2 == AWEFD_CART_TX2, "2-D tile width");
 
 template <int NDIM>
 __device__ __forceinline__ float sponge(float v, int iz, int ix, int iy, const awefd_cart_args &a)
@@ -864,11 +870,13 @@ __global__ __launch_bounds__(WAVE) void k_cart_col(const awefd_cart_args a)
 }
 
 /* The single-grid step kernel: one launch per time step of one model. */
-template <int NOP, int NDIM, bool SSE, bool VD>
-__global__ __launch_bounds__(NTHR0) void k_cart_step(const awefd_cart_args a)
+template <int NOP, int NDIM, bool SSE, bool VD, bool NARROW>
+__global__ __launch_bounds__(step_nw<NARROW>() * WAVE) void k_cart_step(const awefd_cart_args a)
 {
+    static_assert(NDIM == 2 || !NARROW, "narrow tiles are 2-D");
     float *dst = NDIM == 2 && a.Pn ? a.Pn : a.P;
-    cart_step_body<NOP, NDIM, SSE, VD, RX0, NWAVE0>(a, a.P, a.C, a.it, a.trow, blockIdx.x, dst);
+    cart_step_body<NOP, NDIM, SSE, VD, step_rx<NARROW>(), step_nw<NARROW>()>(a, a.P, a.C, a.it, a.trow, blockIdx.x,
+                                                                            dst);
 }
 
 /* The batched 2-D step kernel: one launch advances every shot of a batch one
@@ -1563,11 +1571,21 @@ int launch_step(const awefd_cart_args *a, bool vd, hipStream_t s)
             hipLaunchKernelGGL((k_cart_col<NOP, SSE, false>), dim3(a->nblk + nx), dim3(WAVE), lds, s, *a);
         return (int)hipGetLastError();
     }
-    const int nextract = (a->nr + NTHR0 - 1) / NTHR0;
+    if (NDIM == 2 && a->narrow) {
+        constexpr int nthr = step_nw<true>() * WAVE;
+        const int nextract = (a->nr + nthr - 1) / nthr;
+        if (vd)
+            hipLaunchKernelGGL((k_cart_step<NOP, 2, SSE, true, true>), dim3(a->nblk + nextract), dim3(nthr), 0, s, *a);
+        else
+            hipLaunchKernelGGL((k_cart_step<NOP, 2, SSE, false, true>), dim3(a->nblk + nextract), dim3(nthr), 0, s, *a);
+        return (int)hipGetLastError();
+    }
+    constexpr int nthr = step_nw<false>() * WAVE;
+    const int nextract = (a->nr + nthr - 1) / nthr;
     if (vd)
-        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, true>), dim3(a->nblk + nextract), dim3(NTHR0), 0, s, *a);
+        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, true, false>), dim3(a->nblk + nextract), dim3(nthr), 0, s, *a);
     else
-        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, false>), dim3(a->nblk + nextract), dim3(NTHR0), 0, s, *a);
+        hipLaunchKernelGGL((k_cart_step<NOP, NDIM, SSE, false, false>), dim3(a->nblk + nextract), dim3(nthr), 0, s, *a);
     return (int)hipGetLastError();
 }
 
@@ -1575,7 +1593,8 @@ template <int NOP, int NDIM, bool SSE>
 int occupancy_step()
 {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cart_step<NOP, NDIM, SSE, false>, NTHR0, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_cart_step<NOP, NDIM, SSE, false, false>, NTHR0, 0) !=
+        hipSuccess)
         return 0;
     return nb;
 }
