@@ -509,25 +509,50 @@ __global__ __launch_bounds__(AWEFD_ADJ2_NWAVE * WAVE) void k_adj2d_batch2(const 
 /* dJ/ds on the model grid, summed over the shots in shot order: for each
  * shot, the sum over the padded nodes copied from model node (kz, kx) of
  * dJ/dV * dV/ds, dV/ds = -V v^2 = -V (V / dt^2) (0 where a free surface
- * fixed V = 0).  All shots share the grid. */
+ * fixed V = 0).  All shots share the grid.  An edge node has nb + 1 copies
+ * per side (a corner (nb + 1)^2), added x outer, z inner; the loads of 8
+ * copies go out before their 8 adds, which keep that order. */
 __global__ __launch_bounds__(256) void k_adj2d_chain(const awefd_adj2d_args *__restrict__ shots, int nshot,
                                                      float *gs, int mz, int mx, int nz, int nx, int ld, int nb)
 {
+    constexpr int B = 8;
     const int kz = blockIdx.x * 64 + (threadIdx.x & 63);
     const int kx = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (kz >= mz || kx >= mx) return;
     const int z0 = kz == 0 ? 0 : kz + nb, z1 = kz == mz - 1 ? nz : kz + nb + 1;
     const int x0 = kx == 0 ? 0 : kx + nb, x1 = kx == mx - 1 ? nx : kx + nb + 1;
+    const int ncopy = (z1 - z0) * (x1 - x0);
     float tot = 0.f;
     for (int s = 0; s < nshot; s++) {
         const awefd_adj2d_args &a = shots[s];
+        const float *V = a.V, *gV = a.gV;
+        const float idt2 = a.idt2;
         float acc = 0.f;
-        for (int ix = x0; ix < x1; ix++)
-            for (int iz = z0; iz < z1; iz++) {
+        int ix = x0, iz = z0, j = 0;
+        for (; j + B <= ncopy; j += B) {
+            float v[B], g[B];
+#pragma unroll
+            for (int k = 0; k < B; k++) {
                 const long i = (long)ix * ld + iz;
-                const float v = a.V[i];
-                acc += a.gV[i] * (-v * (v * a.idt2));
+                v[k] = V[i];
+                g[k] = gV[i];
+                if (++iz == z1) {
+                    iz = z0;
+                    ix++;
+                }
             }
+#pragma unroll
+            for (int k = 0; k < B; k++) acc += g[k] * (-v[k] * (v[k] * idt2));
+        }
+        for (; j < ncopy; j++) {
+            const long i = (long)ix * ld + iz;
+            const float v = V[i];
+            acc += gV[i] * (-v * (v * idt2));
+            if (++iz == z1) {
+                iz = z0;
+                ix++;
+            }
+        }
         tot = s == 0 ? acc : tot + acc;
     }
     gs[(long)kx * mz + kz] = tot;
