@@ -510,8 +510,11 @@ __global__ __launch_bounds__(AWEFD_ADJ2_NWAVE * WAVE) void k_adj2d_batch2(const 
  * shot, the sum over the padded nodes copied from model node (kz, kx) of
  * dJ/dV * dV/ds, dV/ds = -V v^2 = -V (V / dt^2) (0 where a free surface
  * fixed V = 0).  All shots share the grid.  An edge node has nb + 1 copies
- * per side (a corner (nb + 1)^2), added x outer, z inner; the loads of 8
- * copies go out before their 8 adds, which keep that order. */
+ * per side (a corner (nb + 1)^2), added x outer, z inner.  A thread takes SG
+ * shots at a time, one accumulator each, and issues the loads of 8 copies of
+ * all SG shots before their adds, which keep each shot's order; the shots'
+ * sums are then added in shot order. */
+template <int SG>
 __global__ __launch_bounds__(256) void k_adj2d_chain(const awefd_adj2d_args *__restrict__ shots, int nshot,
                                                      float *gs, int mz, int mx, int nz, int nx, int ld, int nb)
 {
@@ -523,37 +526,66 @@ __global__ __launch_bounds__(256) void k_adj2d_chain(const awefd_adj2d_args *__r
     const int x0 = kx == 0 ? 0 : kx + nb, x1 = kx == mx - 1 ? nx : kx + nb + 1;
     const int ncopy = (z1 - z0) * (x1 - x0);
     float tot = 0.f;
-    for (int s = 0; s < nshot; s++) {
-        const awefd_adj2d_args &a = shots[s];
-        const float *V = a.V, *gV = a.gV;
-        const float idt2 = a.idt2;
-        float acc = 0.f;
+    for (int s0 = 0; s0 < nshot; s0 += SG) {
+        const int ns = min(SG, nshot - s0);
+        const float *V[SG], *gV[SG];
+        float idt2[SG], acc[SG];
+#pragma unroll
+        for (int q = 0; q < SG; q++) {
+            const awefd_adj2d_args &a = shots[s0 + (q < ns ? q : 0)];
+            V[q] = a.V;
+            gV[q] = a.gV;
+            idt2[q] = a.idt2;
+            acc[q] = 0.f;
+        }
         int ix = x0, iz = z0, j = 0;
         for (; j + B <= ncopy; j += B) {
-            float v[B], g[B];
+            long idx[B];
 #pragma unroll
             for (int k = 0; k < B; k++) {
-                const long i = (long)ix * ld + iz;
-                v[k] = V[i];
-                g[k] = gV[i];
+                idx[k] = (long)ix * ld + iz;
                 if (++iz == z1) {
                     iz = z0;
                     ix++;
                 }
             }
+            float v[SG][B], g[SG][B];
 #pragma unroll
-            for (int k = 0; k < B; k++) acc += g[k] * (-v[k] * (v[k] * idt2));
+            for (int q = 0; q < SG; q++)
+                if (q < ns) {
+#pragma unroll
+                    for (int k = 0; k < B; k++) {
+                        v[q][k] = V[q][idx[k]];
+                        g[q][k] = gV[q][idx[k]];
+                    }
+                }
+#pragma unroll
+            for (int q = 0; q < SG; q++)
+                if (q < ns) {
+#pragma unroll
+                    for (int k = 0; k < B; k++) acc[q] += g[q][k] * (-v[q][k] * (v[q][k] * idt2[q]));
+                }
         }
         for (; j < ncopy; j++) {
             const long i = (long)ix * ld + iz;
-            const float v = V[i];
-            acc += gV[i] * (-v * (v * idt2));
+            float v[SG], g[SG];
+#pragma unroll
+            for (int q = 0; q < SG; q++)
+                if (q < ns) {
+                    v[q] = V[q][i];
+                    g[q] = gV[q][i];
+                }
+#pragma unroll
+            for (int q = 0; q < SG; q++)
+                if (q < ns) acc[q] += g[q] * (-v[q] * (v[q] * idt2[q]));
             if (++iz == z1) {
                 iz = z0;
                 ix++;
             }
         }
-        tot = s == 0 ? acc : tot + acc;
+#pragma unroll
+        for (int q = 0; q < SG; q++)
+            if (q < ns) tot = s0 + q == 0 ? acc[q] : tot + acc[q];
     }
     gs[(long)kx * mz + kz] = tot;
 }
@@ -653,8 +685,12 @@ extern "C" int awefd_launch_adj2d_chain(const awefd_adj2d_args *h, const awefd_a
                                         float *gout, void *stream)
 {
     const dim3 g((h->mz + 63) / 64, (h->mx + 3) / 4);
-    hipLaunchKernelGGL(k_adj2d_chain, g, dim3(256), 0, (hipStream_t)stream, shots, nshot, gout, h->mz, h->mx,
-                       h->nz, h->nx, h->ld, h->nb);
+    if (nshot >= 4)
+        hipLaunchKernelGGL(k_adj2d_chain<4>, g, dim3(256), 0, (hipStream_t)stream, shots, nshot, gout, h->mz, h->mx,
+                           h->nz, h->nx, h->ld, h->nb);
+    else
+        hipLaunchKernelGGL(k_adj2d_chain<1>, g, dim3(256), 0, (hipStream_t)stream, shots, nshot, gout, h->mz, h->mx,
+                           h->nz, h->nx, h->ld, h->nb);
     return (int)hipGetLastError();
 }
 
