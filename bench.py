@@ -269,8 +269,10 @@ def measure_batch(nt, nshot=256, steps=1, tune_steps=20, per_shot=True, fixed=No
     npass, nsingle = (nt - 1) // depth if depth > 1 else 0, nt - ((nt - 1) // depth) * depth if depth > 1 else nt
     bytes_run = (npass * 20 + nsingle * 16) * cells
     achieved = bytes_run / secs / 1e9
-    kern = {1: "k_cart_batch2<2,true,%d,4>", 2: "k_cart_batch_t2<2,true,%d,4>",
-            3: "k_cart_batch_t3<2,true,%d,8>"}[depth] % rx
+    if depth == 3:
+        kern = "k_cart_batch_t3<2,true,%d,%d>" % (rx, 32 // rx)     # 32 x rows per block
+    else:
+        kern = {1: "k_cart_batch2<2,true,%d,4>", 2: "k_cart_batch_t2<2,true,%d,4>"}[depth] % rx
     out = dict(workload="batch2d %d shots nz~U[200,600] nx~U[400,1200] nb=50 nt=%d, %d padded cells"
                         % (nshot, nt, cells),
                shots_per_s=round(batched, 3), gcells_per_s=round(cells * nt / secs / 1e9, 3),

@@ -422,10 +422,11 @@ static int t2_tile(const awefd_batch *b, int *sz, int *sx)
 }
 
 /* Tile geometry of a multi-step pass: depth 2 as above; depth 3 blocks of
- * 8 waves (64 z x 8 rx rows) with outputs 2 nop inside on every side. */
+ * 64 z x AWEFD_T3_ROWS x rows (AWEFD_T3_NWAVE(rx) waves) with outputs 2 nop
+ * inside on every side. */
 static int tn_tile(const awefd_batch *b, int depth, int *sz, int *sx, int *rows, int *halo)
 {
-    const int nw = depth == 3 ? AWEFD_BATCH_NWAVE3 : AWEFD_BATCH_NWAVE;
+    const int nw = depth == 3 ? AWEFD_T3_NWAVE(b->rx) : AWEFD_BATCH_NWAVE;
     *halo = (depth - 1) * b->nop;
     *rows = nw * b->rx;
     *sz = 64 - 2 * *halo;
@@ -531,7 +532,7 @@ static int t2_build(awefd_batch *b, int depth)
      * block table about to be replaced */
     b->graph_ok = b->fwd_ok = b->bwd_ok = 0;
     b->t2_ok = 0;
-    const int nthr = (depth == 3 ? AWEFD_BATCH_NWAVE3 : AWEFD_BATCH_NWAVE) * 64, ns = b->nshot, ng = b->ngroup;
+    const int nthr = (depth == 3 ? AWEFD_T3_NWAVE(b->rx) : AWEFD_BATCH_NWAVE) * 64, ns = b->nshot, ng = b->ngroup;
     if (!b->e0) b->e0 = calloc(ns, sizeof(float *));
     if (!b->e1) b->e1 = calloc(ns, sizeof(float *));
     if (!b->d_inj2) b->d_inj2 = calloc(ns, sizeof(awefd_inj *));

@@ -39,9 +39,16 @@ extern "C" {
 #define AWEFD_COL_PD 3            /* planes its loads run ahead                    */
 #endif
 #define AWEFD_BATCH_NWAVE 4      /* waves per block of the batched 2-D kernel     */
-#ifndef AWEFD_BATCH_NWAVE3
-#define AWEFD_BATCH_NWAVE3 8     /* waves per block of the three-step 2-D pass    */
+/* x rows per block of the three-step 2-D pass: 32, as 4 waves of 8 rows
+ * (rx 8) or 8 waves of 4 rows (rx 4).  On 256 uneven shots, per step: rx 8 x
+ * 4 waves 0.194-0.200 ms, x 6 waves 0.250, x 8 waves (64 rows) 0.203-0.212,
+ * x 12 0.228, x 16 0.302; rx 4 x 8 waves 0.249-0.257, x 4 waves 0.286-0.291
+ * (the steps' barriers wait on every wave of a block; smaller blocks read
+ * more halo). */
+#ifndef AWEFD_T3_ROWS
+#define AWEFD_T3_ROWS 32
 #endif
+#define AWEFD_T3_NWAVE(rx) (AWEFD_T3_ROWS / (rx))
 
 /* One source-injection contribution: node (z, x, y) += amp(src) * w * vel. */
 typedef struct {
@@ -307,7 +314,8 @@ int awefd_launch_cart_batch_t2(const awefd_cart_args *shots, const int *bshot, i
                                int nop, int sse, int rx, void *stream);
 /* Batched 2-D steps it, it+1, it+2 of every shot in one pass (nst = 3), or
  * step it alone (nst = 1) on the same tiles: roles as above (outputs: the
- * second and third steps' fields); rx in {4, 8} with 8 waves per block. */
+ * second and third steps' fields); rx in {4, 8}, AWEFD_T3_NWAVE(rx) waves
+ * per block. */
 int awefd_launch_cart_batch_t3(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int nst,
                                int nop, int sse, int rx, void *stream);
 /* recompute step n of every shot inside its checkpoint segment (out of

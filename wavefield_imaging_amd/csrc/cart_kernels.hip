@@ -1657,10 +1657,10 @@ template <int NOP, bool SSE, bool RC>
 int launch_t3_sse(const awefd_cart_args *shots, const int *bshot, int nblk, int it, int roles, int nst, int rx,
                   hipStream_t s)
 {
-    constexpr int NW = AWEFD_BATCH_NWAVE3;
+    constexpr int N4 = AWEFD_T3_NWAVE(4), N8 = AWEFD_T3_NWAVE(8);
     switch (rx) {
-    case 4: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 4, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
-    case 8: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 8, NW, RC>), dim3(nblk), dim3(NW * WAVE), 0, s, shots, bshot, it, roles, nst); break;
+    case 4: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 4, N4, RC>), dim3(nblk), dim3(N4 * WAVE), 0, s, shots, bshot, it, roles, nst); break;
+    case 8: hipLaunchKernelGGL((k_cart_batch_t3<NOP, SSE, 8, N8, RC>), dim3(nblk), dim3(N8 * WAVE), 0, s, shots, bshot, it, roles, nst); break;
     default: return (int)hipErrorInvalidValue;
     }
     return (int)hipGetLastError();
