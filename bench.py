@@ -32,8 +32,10 @@ launch on its own stream, inside the timed region.  cart2d runs as one
 replayed graph: its time per step (kernel plus the graph's launch gaps, an
 upper bound on the kernel) over the timed region.  batch2d and adjoint2d run
 as graphs too: real bytes of their passes over the timed graphs (20 bytes per
-node for a three-step pass; 60 per node and step of a gradient: forward,
-recompute and the 28-byte adjoint step).
+node for a three-step pass; for a gradient the bytes of its forward,
+checkpoint, recompute and adjoint passes, 32.9 per node and step with
+three-step forward and recompute passes and two-step adjoint passes, 60
+with one step per launch).
 
 cpu_baseline: the reference CPU path restated in C (oracle/liborc_ref.so:
 the 4-wide SSE step with schedule(static,1) at -O3, OpenMP over r for the
